@@ -1,0 +1,251 @@
+/*
+ * mp_abi.h -- C ABI of the MI355X (gfx950) motion-planning hot path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  Device pointers are
+ * HIP device addresses; streams are hipStream_t passed as void* (NULL = the library's
+ * stream).  Every function returns MP_OK (0) or a negative MP_E* code; mp_last_error()
+ * returns a message for the calling thread's last failure.  Nothing here exits the process.
+ *
+ * Paths below are relative to the reference checkout (jih189/isaac_sim_ros2_humble).  Each
+ * entry point names the reference interface it replaces.
+ */
+#ifndef MP_ABI_H
+#define MP_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MP_OK 0
+#define MP_E_INVALID (-1)  /* bad argument / size mismatch (reference: std::runtime_error) */
+#define MP_E_HIP (-2)      /* HIP runtime error (reference: CUDA_CHECK -> exit) */
+#define MP_E_RTC (-3)      /* hipRTC compile error (reference: NVRTC_SAFE_CALL -> exit) */
+#define MP_E_NOMEM (-4)    /* device allocation failed (reference: isValid()==false / nullptr) */
+#define MP_E_CAPACITY (-5) /* obstacle cache / tree / path capacity exceeded */
+#define MP_E_STATE (-6)    /* library not initialised / object in wrong state */
+
+typedef struct mp_robot mp_robot;
+typedef struct mp_scene mp_scene;
+typedef struct mp_server mp_server;
+
+/* ---------------------------------------------------------------------------------------- */
+/* Lifecycle and device plumbing                                                            */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Select the HIP device and create the library stream.  Replaces the implicit context
+ * creation of KernelFunction::create (cudampl/CUDAMPLib/include/NvrtcUtil.h:127-139). */
+int mp_init(int device);
+int mp_shutdown(void);
+const char* mp_last_error(void);
+/* Directory of the on-disk code-object cache (default: $MP_KERNEL_CACHE or
+ * <library dir>/_kcache).  Replaces the CWD "<kernel>.ptx" cache (NvrtcUtil.h:62-125), keyed
+ * by a hash of the generated source, compile options and hipRTC version. */
+int mp_set_cache_dir(const char* dir);
+
+int mp_device_alloc(void** dptr, size_t bytes);
+int mp_device_free(void* dptr);
+int mp_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int mp_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int mp_memset_d8(void* dst, int value, size_t bytes, void* stream);
+int mp_stream_sync(void* stream);
+void* mp_default_stream(void);
+/* Elapsed ms of the most recent launch of the named kernel family on the library stream
+ * ("check", "fk", "edges", "nn", "sample", "plan"), measured with HIP events around it. */
+double mp_last_kernel_ms(const char* family);
+/* Algorithmic flops (FK, sphere transforms, bounding-sphere and pair tests actually evaluated,
+ * at the per-operation costs of the planning cost model) accumulated by the check / cost /
+ * edge kernels since the last reset. */
+int mp_work_flops(uint64_t* out, int reset);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Robot                                                                                    */
+/* ---------------------------------------------------------------------------------------- */
+/* RobotInfo layout (src/foliation_planner/include/foliation_planner/robot_info.hpp:13-220),
+ * as taken by SingleArmSpace (cudampl/CUDAMPLib/include/spaces/SingleArmSpace.h:24-40) and
+ * CPRRTC::RobotSolver (cudampl/CPRRTC/include/RobotSolver.h:84-101). */
+typedef struct {
+    const char* name;
+    int num_links;                 /* L = number of joints (one joint per link, root first) */
+    const int* joint_types;        /* [L] 0 root, 1 revolute, 2 prismatic, 5 fixed */
+    const double* joint_origins;   /* [L][16] row-major Isometry3d::matrix() */
+    const double* joint_axes;      /* [L][3] */
+    const int* parents;            /* [L] parent link index, -1 for the root */
+    const int* active;             /* [L] 1 if the joint is a planning DOF */
+    const float* lower;            /* [L] (active joints) */
+    const float* upper;            /* [L] */
+    const float* defaults;         /* [L] values of inactive joints */
+    int num_spheres;               /* S */
+    const int* sphere_link;        /* [S] */
+    const float* sphere_pos;       /* [S][3] centre in link frame */
+    const float* sphere_radius;    /* [S] */
+    const unsigned char* collision_enabled; /* [L][L] link-pair self-collision enable map */
+} mp_robot_desc;
+
+/* Generate robot-specialised HIP source (unrolled FK, self-collision pair set) and compile it
+ * with hipRTC for gfx950 (cached).  Replaces SingleArmSpace's ctor JIT build
+ * (cudampl/CUDAMPLib/src/spaces/SingleArmSpace.cu:10-161) and RobotSolver's
+ * (cudampl/CPRRTC/src/RobotSolver.cu:37-141). */
+int mp_robot_create(const mp_robot_desc* desc, mp_robot** out);
+int mp_robot_destroy(mp_robot* robot);
+/* dof, spheres, self pairs, links */
+int mp_robot_info(const mp_robot* robot, int* dof, int* num_spheres, int* num_pairs, int* num_links);
+/* Generated source text (for inspection); returns its length, copies up to cap bytes. */
+size_t mp_robot_source(const mp_robot* robot, char* buf, size_t cap);
+/* Compile-only: generate + hipRTC-compile + store in the cache, no device needed.  Used by
+ * the build step to pre-populate the cache for the bundled robots. */
+int mp_robot_precompile(const mp_robot_desc* desc);
+/* Compile-only for a robot set (the module a mixed batch or a server uses). */
+int mp_precompile(const mp_robot_desc* const* descs, int n);
+/* Generated module source for a robot set without compiling it (no device needed); returns
+ * the length, copies up to cap-1 bytes plus a terminating NUL. */
+size_t mp_generate_source(const mp_robot_desc* const* descs, int n, char* buf, size_t cap);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Scene                                                                                    */
+/* ---------------------------------------------------------------------------------------- */
+/* Obstacle records (float32, base frame):
+ *   spheres   [ns][4]   x, y, z, radius                 EnvConstraintSphere (EnvConstraintSphere.cu:10-33),
+ *                                                       CPRRTC Sphere POD (cudampl/CPRRTC/include/Obstacle.h:4-10)
+ *   boxes     [nb][18]  pos(3), R(9 row-major), min(3), max(3)
+ *                                                       EnvConstraintCuboid (EnvConstraintCuboid.cu:36-79)
+ *   cylinders [nc][14]  pos(3), R(9), radius, height    EnvConstraintCylinder (EnvConstraintCylinder.cu:50-100)
+ *   capsules  [ncap][7] p1(3), p2(3), radius            pRRTC Capsule (pRRTC/src/planning/utils.cuh:113-127)
+ * The scene is resident in HBM until destroyed. */
+int mp_scene_create(const float* spheres, int ns, const float* boxes, int nb, const float* cylinders, int nc,
+                    const float* capsules, int ncap, mp_scene** out);
+/* Replace the obstacles (RobotSolver::updateEnvObstacle, RobotSolver.cu:179-204); the counts
+ * must fit the capacities given at creation (MP_E_CAPACITY otherwise, where the reference
+ * printed and returned). */
+int mp_scene_update(mp_scene* scene, const float* spheres, int ns, const float* boxes, int nb,
+                    const float* cylinders, int nc, const float* capsules, int ncap);
+int mp_scene_destroy(mp_scene* scene);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Data-parallel primitives (device pointers, asynchronous on `stream`)                     */
+/* ---------------------------------------------------------------------------------------- */
+/* Sphere centres of n configurations.  q: [n][dof] active joints; centres: [n][S][3].
+ * Replaces SingleArmStates::update (SingleArmStates.cu:696-723): FK kernel + sphere kernel
+ * fused, no link poses written. */
+int mp_fk_spheres(const mp_robot* robot, const float* d_q, int64_t n, float* d_centres, void* stream);
+
+/* Collision verdicts of n configurations: verdict[i] = 1 iff every constraint cost is 0
+ * (self pairs, all obstacle types, joint bounds).  Early-out per wave.  d_centres optional.
+ * Replaces states->update(); space->checkStates(states, feasible)
+ * (SingleArmSpace.cu:1572-1602, one cost kernel per constraint + calculateTotalCosts). */
+int mp_check_states(const mp_robot* robot, const mp_scene* scene, const float* d_q, int64_t n,
+                    uint8_t* d_verdict, float* d_centres, void* stream);
+
+/* Full-evaluation total cost per configuration (no early-out), the value
+ * BaseStates::getTotalCostsHost returns after checkStates (States.cu:5-63). */
+int mp_state_costs(const mp_robot* robot, const mp_scene* scene, const float* d_q, int64_t n,
+                   float* d_cost, uint8_t* d_verdict, void* stream);
+
+/* Edge checks with checkMotions semantics (SingleArmSpace.cu:646-848): steps =
+ * floor(d/res)+1, states q0 + dir*i (i < steps-1) then q1; ok[e] = all valid, len[e] = d.
+ * No host round trip, no temporary allocation. */
+int mp_check_edges(const mp_robot* robot, const mp_scene* scene, const float* d_q0, const float* d_q1,
+                   int m, float resolution, uint8_t* d_ok, float* d_len, void* stream);
+
+/* Nearest neighbour (squared L2 over dof, ties -> lowest index) of nq queries [nq][dof] in a
+ * structure-of-arrays tree [dof][capacity] holding n_nodes nodes.  Replaces
+ * find_the_nearest_neighbors_kernel (SingleArmStates.cu:1265-1431). */
+int mp_nn_argmin(const float* d_tree_soa, int n_nodes, int capacity, int dof, const float* d_q, int nq,
+                 int* d_idx, void* stream);
+
+/* Uniform samples within the robot's active-joint bounds from a Philox4x32-10 stream:
+ * q[i] = sample(seed, offset + i).  Replaces initCurand + sample_kernel
+ * (SingleArmSpace.cu:184-217, 251-322). */
+int mp_sample(const mp_robot* robot, uint64_t seed, uint64_t offset, int64_t n, float* d_q, void* stream);
+
+/* Steering: to[i] <- from[i] + (max/d)(to[i]-from[i]) when d > max (interpolate_kernel,
+ * SingleArmSpace.cu:219-249). */
+int mp_steer(const float* d_from, float* d_to, int n, int dof, float max_distance, void* stream);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Planning                                                                                 */
+/* ---------------------------------------------------------------------------------------- */
+typedef struct {
+    float resolution;      /* edge check resolution, rad (RobotSolver default 0.02) */
+    int max_step;          /* configurations per extend/connect chunk (CPRRTC 32) */
+    int max_iterations;    /* samples per query (CPRRTC max_iterations_ = 10000) */
+    int tree_capacity;     /* nodes per tree (power of two recommended) */
+    int waves;             /* persistent grid size in waves (0 = tuned default) */
+    int max_path;          /* waypoint capacity per query */
+    uint64_t seed;         /* sample stream seed; query k uses seed + k */
+    int exclusive;         /* 1: one wave grows each query (deterministic per seed, identical
+                              alone and in a batch); 0: all waves share all queries */
+} mp_rrtc_params;
+
+/* Defaults: resolution 0.02, max_step 32, max_iterations 10000, tree_capacity 65536,
+ * waves 0, max_path 4096, seed 0, exclusive 0. */
+void mp_rrtc_default_params(mp_rrtc_params* p);
+
+#define MP_PLAN_RUNNING 0
+#define MP_PLAN_SOLVED 1
+#define MP_PLAN_EXHAUSTED 2   /* max_iterations samples used (reference: returns {}) */
+#define MP_PLAN_TREE_FULL 3
+#define MP_PLAN_INVALID 4     /* start or goal in collision / out of bounds */
+
+typedef struct {
+    int status;            /* MP_PLAN_* */
+    int num_waypoints;     /* waypoints written to `waypoints` */
+    float* waypoints;      /* caller buffer [capacity][dof], host memory */
+    int capacity;
+    int iterations;        /* samples consumed */
+    int start_tree_size;
+    int goal_tree_size;
+    int64_t configs_checked;
+    double latency_ms;     /* submit -> result on host */
+    double work_flops;     /* algorithmic flops of the collision checks run for this query */
+} mp_path;
+
+/* One query, one launch: the CPRRTC::RobotSolver::solve equivalent (RobotSolver.cu:348-473):
+ * all waves of the grid grow the two trees of this query; the path is start ... goal
+ * (start-tree half root->connection, goal-tree half connection->root, RobotSolver.cu:267-346). */
+int mp_rrtc_solve(mp_robot* robot, const mp_scene* scene, const float* start, const float* goal,
+                  const mp_rrtc_params* params, mp_path* out);
+
+/* nq independent queries (robots may differ) in ONE persistent launch with per-query work
+ * queues; robots[k], scenes[k], starts[k] / goals[k] ([dof_k] floats at offset k*16). */
+int mp_rrtc_solve_batch(int nq, mp_robot* const* robots, const mp_scene* const* scenes, const float* starts16,
+                        const float* goals16, const mp_rrtc_params* params, mp_path* out);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Serving: continuous batching                                                             */
+/* ---------------------------------------------------------------------------------------- */
+typedef struct {
+    int num_robots;
+    mp_robot* const* robots;   /* robots the server can plan for (one module for the set) */
+    size_t tree_memory_bytes;  /* device memory for tree buffers; admission is by free memory */
+    int tree_capacity;         /* nodes per tree */
+    int max_active;            /* upper bound on concurrently active queries */
+    int step_iterations;       /* extend-connect iterations per step launch (0 = auto) */
+    int waves;                 /* grid size in waves (0 = auto) */
+    int use_graph;             /* capture the step in a hipGraph and replay it */
+    mp_rrtc_params params;
+} mp_server_params;
+
+typedef struct {
+    int64_t submitted, completed, solved;
+    int active, queued;
+    double p50_ms, p99_ms, mean_ms, throughput_qps;
+    int64_t steps;
+} mp_server_stats;
+
+/* A server thread owns the device; submit/poll are thread-safe. */
+int mp_server_create(const mp_server_params* params, mp_server** out);
+int mp_server_submit(mp_server* s, int robot_index, const mp_scene* scene, const float* start, const float* goal,
+                     uint64_t seed, int64_t* ticket);
+/* Returns 1 and fills `out` when the query has finished, 0 if still pending. */
+int mp_server_poll(mp_server* s, int64_t ticket, mp_path* out);
+int mp_server_wait_idle(mp_server* s, double timeout_s);
+int mp_server_stats_get(mp_server* s, mp_server_stats* out);
+int mp_server_destroy(mp_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

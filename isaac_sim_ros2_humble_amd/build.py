@@ -1,0 +1,73 @@
+"""Build the native library in-tree: isaac_sim_ros2_humble_amd/libmpamd.so.
+
+Steps: embed the kernel template (csrc/mp_kernels.hip.tmpl) into the host library as a string,
+compile the C++ host sources against the HIP runtime + hipRTC, then (optionally) pre-populate
+the code-object cache by hipRTC-compiling the gfx950 modules for the bundled robots, so a fresh
+process on the GPU finds them on disk.  No GPU is needed for any step.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+REPO = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libmpamd.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+SOURCES = ["mp_capi.cpp", "codegen.cpp", "rtc.cpp", "plan.cpp", "server.cpp"]
+
+
+def embed_template():
+    src = open(os.path.join(CSRC, "mp_kernels.hip.tmpl")).read()
+    out = os.path.join(CSRC, "_kernels_src.inc")
+    # split into chunks: some compilers cap single string literal length
+    chunks = [src[i:i + 12000] for i in range(0, len(src), 12000)]
+    body = "\n".join('R"MPKSRC(' + c + ')MPKSRC"' for c in chunks)
+    new = body + "\n"
+    if not os.path.exists(out) or open(out).read() != new:
+        with open(out, "w") as f:
+            f.write(new)
+
+
+def compile_lib(verbose=False):
+    embed_template()
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-Wall",
+           "-Wno-unused-function", "-Wno-unused-result", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(REPO, "include"), "-I" + CSRC,
+           "-I" + os.path.join(ROCM, "include")]
+    cmd += [os.path.join(CSRC, s) for s in SOURCES]
+    cmd += ["-o", LIB + ".tmp", "-L" + os.path.join(ROCM, "lib"), "-Wl,-rpath," + os.path.join(ROCM, "lib"),
+            "-lamdhip64", "-lhiprtc", "-ldl", "-lpthread"]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.check_call(cmd)
+    os.replace(LIB + ".tmp", LIB)
+
+
+def _precompile_one(names):
+    sys.path.insert(0, REPO)
+    from isaac_sim_ros2_humble_amd import native
+    from isaac_sim_ros2_humble_amd.model import load_robot
+    native.precompile([load_robot(n) for n in names])
+    return names
+
+
+def precompile_bundled():
+    """hipRTC-compile the modules for the bundled robots into the on-disk cache (one process per
+    robot set, in parallel)."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    sets = [(), ("fr3",), ("fetch",), ("fr3", "fetch")]
+    with cf.ProcessPoolExecutor(len(sets), mp_context=mp.get_context("spawn")) as ex:  # noqa: SIM117
+        for names in ex.map(_precompile_one, sets):
+            pass
+
+
+def build(precompile=True, verbose=False):
+    compile_lib(verbose)
+    if precompile:
+        precompile_bundled()
+
+
+if __name__ == "__main__":
+    build(precompile="--no-precompile" not in sys.argv, verbose=True)

@@ -1,0 +1,97 @@
+// Host mirrors of the planner's device structures (layouts must match the kernel template).
+#pragma once
+
+#include <functional>
+
+#include "mp_internal.h"
+
+#define MP_SPH_W_H 4
+#define MP_BOX_W_H 20
+#define MP_CYL_W_H 16
+#define MP_CAP_W_H 8
+
+namespace mp {
+
+struct SceneDevHost {
+    int ns, nb, nc, ncap;
+    const float* sph;
+    const float* box;
+    const float* cyl;
+    const float* cap;
+};
+
+struct QueryDevHost {
+    SceneDevHost scene;
+    float* tree[2];
+    int* parent[2];
+    float* path;
+    uint64_t seed;
+    int robot, dof, cap, max_iters;
+    int max_path, pad0, pad1, pad2;
+    float start[16];
+    float goal[16];
+};
+static_assert(sizeof(QueryDevHost) == 256, "QueryDev layout");
+
+struct QueryStateHost {
+    int alloc[2];
+    int pub[2];
+    int sample_ctr;
+    int status;
+    int conn[2];
+    int path_len;
+    int iters;
+    uint64_t checked;
+    uint64_t flops;
+};
+static_assert(sizeof(QueryStateHost) == 56, "QueryState layout");
+
+// control block in device memory: [0] ticket, [1] remaining, [2] n_active, [3] budget,
+// [4] first new query (index into the active list), [5] number of new queries
+struct PlanCtlHost {
+    QueryDevHost* q;
+    QueryStateHost* st;
+    const int* active;
+    int* ctl;
+    float res;
+    int max_step;
+};
+
+hipStream_t lib_stream();
+int lib_num_cus();
+void launch(hipFunction_t f, unsigned grid, unsigned block, hipStream_t s, void** args, const char* family);
+SceneDevHost scene_view(const mp_scene* sc);
+void plan_release_workspaces();
+
+// Device buffers for `nslots` query slots: two SoA trees, two parent arrays and a path buffer
+// per slot, the query descriptors / states, the active list and the control block.
+struct SlotPool {
+    int nslots = 0, cap = 0, dof_max = 0, max_path = 0;
+    float* trees = nullptr;   // [nslots][2][dof_max][cap]
+    int* parents = nullptr;   // [nslots][2][cap]
+    float* paths = nullptr;   // [nslots][max_path][dof_max]
+    QueryDevHost* dq = nullptr;
+    QueryStateHost* dst = nullptr;
+    int* dactive = nullptr;
+    int* dctl = nullptr;      // control block (8 ints)
+    int* hpinned = nullptr;   // pinned staging: control block (8) + active list (nslots)
+    QueryStateHost* hst_pinned = nullptr;
+    std::vector<QueryDevHost> hq;
+
+    SlotPool(int nslots, int cap, int dof_max, int max_path);
+    ~SlotPool();
+    size_t bytes_per_slot() const;
+    void fill_query(int slot, int robot_slot, int dof, const mp_scene* scene, const float* start, const float* goal,
+                    uint64_t seed, int max_iters);
+    PlanCtlHost ctl(float res, int max_step) const;
+};
+
+size_t slot_bytes(int cap, int dof_max, int max_path);
+
+// Runs f, converting exceptions to an ABI status code (message in mp_last_error()).
+int guard_call(const std::function<void()>& f);
+void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params& p, int waves, hipStream_t s);
+void harvest(SlotPool& pool, int slot, int dof, const QueryStateHost& st, mp_path* out, hipStream_t s);
+int default_waves(hipFunction_t plan_fn);
+
+}  // namespace mp

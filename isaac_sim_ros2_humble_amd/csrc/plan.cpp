@@ -1,0 +1,278 @@
+// Planner host side: slot pools, the plain one-query-per-launch solve (the
+// CPRRTC::RobotSolver::solve equivalent, cudampl/CPRRTC/src/RobotSolver.cu:348-473) and the
+// batched persistent solve (many queries, one launch).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <memory>
+
+#include "mp_plan.h"
+
+using mp::Error;
+
+namespace mp {
+
+size_t slot_bytes(int cap, int dof_max, int max_path) {
+    return (size_t)2 * dof_max * cap * sizeof(float) + (size_t)2 * cap * sizeof(int) +
+           (size_t)max_path * dof_max * sizeof(float);
+}
+
+SlotPool::SlotPool(int ns, int c, int d, int mp_) : nslots(ns), cap(c), dof_max(d), max_path(mp_) {
+    try {
+        MP_HIP(hipMalloc(&trees, (size_t)nslots * 2 * dof_max * cap * sizeof(float)));
+        MP_HIP(hipMalloc(&parents, (size_t)nslots * 2 * cap * sizeof(int)));
+        MP_HIP(hipMalloc(&paths, (size_t)nslots * max_path * dof_max * sizeof(float)));
+        MP_HIP(hipMalloc(&dq, (size_t)nslots * sizeof(QueryDevHost)));
+        MP_HIP(hipMalloc(&dst, (size_t)nslots * sizeof(QueryStateHost)));
+        MP_HIP(hipMalloc(&dactive, (size_t)nslots * sizeof(int)));
+        MP_HIP(hipMalloc(&dctl, 16 * sizeof(int)));
+        MP_HIP(hipHostMalloc(&hpinned, (size_t)(16 + nslots) * sizeof(int), hipHostMallocDefault));
+        MP_HIP(hipHostMalloc(&hst_pinned, (size_t)nslots * sizeof(QueryStateHost), hipHostMallocDefault));
+        MP_HIP(hipMemset(dst, 0, (size_t)nslots * sizeof(QueryStateHost)));
+    } catch (...) {
+        this->~SlotPool();
+        throw;
+    }
+    hq.resize(nslots);
+    std::memset(hq.data(), 0, hq.size() * sizeof(QueryDevHost));
+}
+
+SlotPool::~SlotPool() {
+    if (trees) hipFree(trees);
+    if (parents) hipFree(parents);
+    if (paths) hipFree(paths);
+    if (dq) hipFree(dq);
+    if (dst) hipFree(dst);
+    if (dactive) hipFree(dactive);
+    if (dctl) hipFree(dctl);
+    if (hpinned) hipHostFree(hpinned);
+    if (hst_pinned) hipHostFree(hst_pinned);
+    trees = nullptr;
+    parents = nullptr;
+    paths = nullptr;
+    dq = nullptr;
+    dst = nullptr;
+    dactive = nullptr;
+    dctl = nullptr;
+    hpinned = nullptr;
+    hst_pinned = nullptr;
+}
+
+size_t SlotPool::bytes_per_slot() const { return slot_bytes(cap, dof_max, max_path); }
+
+void SlotPool::fill_query(int slot, int robot_slot, int dof, const mp_scene* scene, const float* start,
+                          const float* goal, uint64_t seed, int max_iters) {
+    QueryDevHost& q = hq[slot];
+    std::memset(&q, 0, sizeof(q));
+    q.scene = scene_view(scene);
+    for (int t = 0; t < 2; t++) {
+        q.tree[t] = trees + ((size_t)slot * 2 + t) * dof_max * cap;
+        q.parent[t] = parents + ((size_t)slot * 2 + t) * cap;
+    }
+    q.path = paths + (size_t)slot * max_path * dof_max;
+    q.seed = seed;
+    q.robot = robot_slot;
+    q.dof = dof;
+    q.cap = cap;
+    q.max_iters = max_iters;
+    q.max_path = max_path;
+    for (int j = 0; j < dof; j++) {
+        q.start[j] = start[j];
+        q.goal[j] = goal[j];
+    }
+}
+
+PlanCtlHost SlotPool::ctl(float res, int max_step) const {
+    PlanCtlHost c;
+    c.q = dq;
+    c.st = dst;
+    c.active = dactive;
+    c.ctl = dctl;
+    c.res = res;
+    c.max_step = max_step;
+    return c;
+}
+
+int default_waves(hipFunction_t plan_fn) {
+    int per_cu = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, plan_fn, 64, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 4;
+    return std::max(1, per_cu) * std::max(lib_num_cus(), 1);
+}
+
+namespace {
+std::mutex g_pool_mu;
+std::map<std::string, std::unique_ptr<SlotPool>> g_pools;
+
+SlotPool& pool_for(const char* tag, int nslots, int cap, int dof_max, int max_path) {
+    std::string k = std::string(tag) + ":" + std::to_string(nslots) + ":" + std::to_string(cap) + ":" +
+                    std::to_string(dof_max) + ":" + std::to_string(max_path);
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    auto it = g_pools.find(k);
+    if (it != g_pools.end()) return *it->second;
+    // keep at most a few pools alive
+    if (g_pools.size() >= 4) {
+        hipStreamSynchronize(lib_stream());
+        g_pools.clear();
+    }
+    auto p = std::make_unique<SlotPool>(nslots, cap, dof_max, max_path);
+    auto& ref = *p;
+    g_pools.emplace(k, std::move(p));
+    return ref;
+}
+
+void check_params(const mp_rrtc_params* p) {
+    if (!p) throw Error(MP_E_INVALID, "null params");
+    if (!(p->resolution > 0.0f)) throw Error(MP_E_INVALID, "resolution must be > 0");
+    if (p->max_step < 1 || p->max_step > 64) throw Error(MP_E_INVALID, "max_step must be in [1,64]");
+    if (p->max_iterations < 1) throw Error(MP_E_INVALID, "max_iterations must be >= 1");
+    if (p->tree_capacity < 2 || p->tree_capacity > (1 << 22)) throw Error(MP_E_INVALID, "tree_capacity out of range");
+    if (p->max_path < 2) throw Error(MP_E_INVALID, "max_path must be >= 2");
+    if (p->waves < 0) throw Error(MP_E_INVALID, "waves must be >= 0");
+}
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+void plan_release_workspaces() {
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    g_pools.clear();
+}
+
+// Runs the queries already described in pool.hq[0..nq) to completion in one persistent launch.
+void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params& p, int waves, hipStream_t s) {
+    MP_HIP(hipMemcpyAsync(pool.dq, pool.hq.data(), (size_t)nq * sizeof(QueryDevHost), hipMemcpyHostToDevice, s));
+    int* h = pool.hpinned;
+    h[0] = 0;       // ticket
+    h[1] = nq;      // running
+    h[2] = nq;      // active-list length
+    h[3] = 0;       // no ticket budget
+    h[4] = 0;       // first new
+    h[5] = nq;      // new count
+    h[6] = p.exclusive ? 1 : 0;
+    for (int k = 0; k < nq; k++) h[16 + k] = k;
+    MP_HIP(hipMemcpyAsync(pool.dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, s));
+    MP_HIP(hipMemcpyAsync(pool.dactive, h + 16, (size_t)nq * sizeof(int), hipMemcpyHostToDevice, s));
+    PlanCtlHost ctl = pool.ctl(p.resolution, p.max_step);
+    void* args[] = {&ctl};
+    launch(mod.fn("mp_plan_init"), (unsigned)nq, 64, s, args, "plan_init");
+    launch(mod.fn("mp_plan"), (unsigned)waves, 64, s, args, "plan");
+    MP_HIP(hipMemcpyAsync(pool.hst_pinned, pool.dst, (size_t)nq * sizeof(QueryStateHost), hipMemcpyDeviceToHost, s));
+    MP_HIP(hipStreamSynchronize(s));
+}
+
+void harvest(SlotPool& pool, int slot, int dof, const QueryStateHost& st, mp_path* out, hipStream_t s) {
+    out->status = st.status < 0 ? MP_PLAN_RUNNING : st.status;
+    out->iterations = std::min(st.sample_ctr, pool.hq[slot].max_iters);
+    out->start_tree_size = std::min(st.pub[0], pool.cap);
+    out->goal_tree_size = std::min(st.pub[1], pool.cap);
+    out->configs_checked = (int64_t)st.checked;
+    out->work_flops = (double)st.flops;
+    out->num_waypoints = 0;
+    if (st.status == MP_PLAN_SOLVED) {
+        if (st.path_len < 0) {
+            out->status = MP_PLAN_TREE_FULL;  // path longer than max_path
+            out->num_waypoints = -st.path_len;
+            return;
+        }
+        int n = std::min(st.path_len, out->capacity);
+        if (n > 0 && out->waypoints) {
+            // device path rows are dof_max wide; copy row-compacted
+            if (dof == pool.dof_max) {
+                MP_HIP(hipMemcpyAsync(out->waypoints, pool.hq[slot].path, (size_t)n * dof * sizeof(float),
+                                      hipMemcpyDeviceToHost, s));
+            } else {
+                MP_HIP(hipMemcpy2DAsync(out->waypoints, dof * sizeof(float), pool.hq[slot].path, dof * sizeof(float),
+                                        dof * sizeof(float), n, hipMemcpyDeviceToHost, s));
+            }
+        }
+        out->num_waypoints = n;
+    }
+}
+
+}  // namespace mp
+
+namespace mp {
+namespace {
+void solve_impl(const char* tag, int nq, mp_robot* const* robots, const mp_scene* const* scenes, const float* starts16,
+                const float* goals16, const mp_rrtc_params* params, mp_path* out, int default_wave_count) {
+    if (lib_stream() == nullptr) throw Error(MP_E_STATE, "mp_init() has not been called");
+    check_params(params);
+    if (nq < 1 || !robots || !scenes || !starts16 || !goals16 || !out) throw Error(MP_E_INVALID, "bad batch arguments");
+    double t0 = now_ms();
+    // robot set -> module (robots in order of first appearance)
+    std::vector<mp_robot*> uniq;
+    std::vector<int> slot_of(nq);
+    int dof_max = 1;
+    for (int k = 0; k < nq; k++) {
+        if (!robots[k] || !scenes[k]) throw Error(MP_E_INVALID, "null robot or scene in batch");
+        int s = -1;
+        for (size_t u = 0; u < uniq.size(); u++)
+            if (uniq[u] == robots[k] || uniq[u]->key == robots[k]->key) s = (int)u;
+        if (s < 0) {
+            s = (int)uniq.size();
+            uniq.push_back(robots[k]);
+        }
+        slot_of[k] = s;
+        dof_max = std::max(dof_max, robots[k]->model.dof);
+    }
+    std::shared_ptr<Module> mod;
+    if (uniq.size() == 1) {
+        mod = uniq[0]->module;
+    } else {
+        std::vector<const RobotModel*> ms;
+        for (auto* r : uniq) ms.push_back(&r->model);
+        mod = get_module(ms);
+    }
+    hipStream_t s = lib_stream();
+    SlotPool& pool = pool_for(tag, nq, params->tree_capacity, dof_max, params->max_path);
+    for (int k = 0; k < nq; k++)
+        pool.fill_query(k, slot_of[k], robots[k]->model.dof, scenes[k], starts16 + 16 * k, goals16 + 16 * k,
+                        params->seed + (uint64_t)k, params->max_iterations);
+    int waves = params->waves > 0 ? params->waves : (default_wave_count > 0 ? default_wave_count : default_waves(mod->fn("mp_plan")));
+    if (params->exclusive) waves = nq;
+    run_to_completion(*mod, pool, nq, *params, waves, s);
+    for (int k = 0; k < nq; k++) harvest(pool, k, robots[k]->model.dof, pool.hst_pinned[k], &out[k], s);
+    MP_HIP(hipStreamSynchronize(s));
+    double lat = now_ms() - t0;
+    for (int k = 0; k < nq; k++) out[k].latency_ms = lat;
+}
+}  // namespace
+}  // namespace mp
+
+extern "C" {
+
+void mp_rrtc_default_params(mp_rrtc_params* p) {
+    if (!p) return;
+    p->resolution = 0.02f;
+    p->max_step = 32;
+    p->max_iterations = 10000;
+    p->tree_capacity = 65536;
+    p->waves = 0;
+    p->max_path = 4096;
+    p->seed = 0;
+    p->exclusive = 0;
+}
+
+int mp_rrtc_solve(mp_robot* robot, const mp_scene* scene, const float* start, const float* goal,
+                  const mp_rrtc_params* params, mp_path* out) {
+    return mp::guard_call([&] {
+        if (!robot || !start || !goal) throw Error(MP_E_INVALID, "null robot/start/goal");
+        float s16[16] = {0}, g16[16] = {0};
+        std::memcpy(s16, start, robot->model.dof * sizeof(float));
+        std::memcpy(g16, goal, robot->model.dof * sizeof(float));
+        // plain mode: one query per launch on a CPRRTC-sized grid (512 waves) unless given
+        mp::solve_impl("plain", 1, &robot, &scene, s16, g16, params, out, 512);
+    });
+}
+
+int mp_rrtc_solve_batch(int nq, mp_robot* const* robots, const mp_scene* const* scenes, const float* starts16,
+                        const float* goals16, const mp_rrtc_params* params, mp_path* out) {
+    return mp::guard_call([&] { mp::solve_impl("batch", nq, robots, scenes, starts16, goals16, params, out, 0); });
+}
+
+}  // extern "C"

@@ -1,0 +1,374 @@
+"""ctypes binding of the native library (libmpamd.so, C ABI in include/mp_abi.h).
+
+Loading fails loudly when the library has not been built: there is no Python fallback for any
+operation of the hot path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmpamd.so")
+
+MP_OK = 0
+PLAN_STATUS = {0: "running", 1: "solved", 2: "exhausted", 3: "tree_full", 4: "invalid"}
+
+
+class MPError(RuntimeError):
+    pass
+
+
+class RobotDesc(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("num_links", C.c_int), ("joint_types", C.c_void_p),
+                ("joint_origins", C.c_void_p), ("joint_axes", C.c_void_p), ("parents", C.c_void_p),
+                ("active", C.c_void_p), ("lower", C.c_void_p), ("upper", C.c_void_p), ("defaults", C.c_void_p),
+                ("num_spheres", C.c_int), ("sphere_link", C.c_void_p), ("sphere_pos", C.c_void_p),
+                ("sphere_radius", C.c_void_p), ("collision_enabled", C.c_void_p)]
+
+
+class RRTCParams(C.Structure):
+    _fields_ = [("resolution", C.c_float), ("max_step", C.c_int), ("max_iterations", C.c_int),
+                ("tree_capacity", C.c_int), ("waves", C.c_int), ("max_path", C.c_int), ("seed", C.c_uint64),
+                ("exclusive", C.c_int)]
+
+
+class Path(C.Structure):
+    _fields_ = [("status", C.c_int), ("num_waypoints", C.c_int), ("waypoints", C.c_void_p), ("capacity", C.c_int),
+                ("iterations", C.c_int), ("start_tree_size", C.c_int), ("goal_tree_size", C.c_int),
+                ("configs_checked", C.c_int64), ("latency_ms", C.c_double), ("work_flops", C.c_double)]
+
+
+class ServerParams(C.Structure):
+    _fields_ = [("num_robots", C.c_int), ("robots", C.c_void_p), ("tree_memory_bytes", C.c_size_t),
+                ("tree_capacity", C.c_int), ("max_active", C.c_int), ("step_iterations", C.c_int), ("waves", C.c_int),
+                ("use_graph", C.c_int), ("params", RRTCParams)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("solved", C.c_int64), ("active", C.c_int),
+                ("queued", C.c_int), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
+                ("throughput_qps", C.c_double), ("steps", C.c_int64)]
+
+
+# name -> (restype, argtypes)
+P = C.c_void_p
+_SIGS = {
+    "mp_init": (C.c_int, [C.c_int]),
+    "mp_shutdown": (C.c_int, []),
+    "mp_last_error": (C.c_char_p, []),
+    "mp_set_cache_dir": (C.c_int, [C.c_char_p]),
+    "mp_device_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "mp_device_free": (C.c_int, [P]),
+    "mp_memcpy_h2d": (C.c_int, [P, P, C.c_size_t, P]),
+    "mp_memcpy_d2h": (C.c_int, [P, P, C.c_size_t, P]),
+    "mp_memset_d8": (C.c_int, [P, C.c_int, C.c_size_t, P]),
+    "mp_stream_sync": (C.c_int, [P]),
+    "mp_default_stream": (P, []),
+    "mp_last_kernel_ms": (C.c_double, [C.c_char_p]),
+    "mp_work_flops": (C.c_int, [C.POINTER(C.c_uint64), C.c_int]),
+    "mp_robot_create": (C.c_int, [C.POINTER(RobotDesc), C.POINTER(C.c_void_p)]),
+    "mp_robot_destroy": (C.c_int, [P]),
+    "mp_robot_info": (C.c_int, [P, P, P, P, P]),
+    "mp_robot_source": (C.c_size_t, [P, P, C.c_size_t]),
+    "mp_robot_precompile": (C.c_int, [C.POINTER(RobotDesc)]),
+    "mp_precompile": (C.c_int, [P, C.c_int]),
+    "mp_generate_source": (C.c_size_t, [P, C.c_int, P, C.c_size_t]),
+    "mp_scene_create": (C.c_int, [P, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int, C.POINTER(C.c_void_p)]),
+    "mp_scene_update": (C.c_int, [P, P, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int]),
+    "mp_scene_destroy": (C.c_int, [P]),
+    "mp_fk_spheres": (C.c_int, [P, P, C.c_int64, P, P]),
+    "mp_check_states": (C.c_int, [P, P, P, C.c_int64, P, P, P]),
+    "mp_state_costs": (C.c_int, [P, P, P, C.c_int64, P, P, P]),
+    "mp_check_edges": (C.c_int, [P, P, P, P, C.c_int, C.c_float, P, P, P]),
+    "mp_nn_argmin": (C.c_int, [P, C.c_int, C.c_int, C.c_int, P, C.c_int, P, P]),
+    "mp_sample": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_int64, P, P]),
+    "mp_steer": (C.c_int, [P, P, C.c_int, C.c_int, C.c_float, P]),
+    "mp_rrtc_default_params": (None, [C.POINTER(RRTCParams)]),
+    "mp_rrtc_solve": (C.c_int, [P, P, P, P, C.POINTER(RRTCParams), C.POINTER(Path)]),
+    "mp_rrtc_solve_batch": (C.c_int, [C.c_int, P, P, P, P, C.POINTER(RRTCParams), P]),
+    "mp_server_create": (C.c_int, [C.POINTER(ServerParams), C.POINTER(C.c_void_p)]),
+    "mp_server_submit": (C.c_int, [P, C.c_int, P, P, P, C.c_uint64, C.POINTER(C.c_int64)]),
+    "mp_server_poll": (C.c_int, [P, C.c_int64, C.POINTER(Path)]),
+    "mp_server_wait_idle": (C.c_int, [P, C.c_double]),
+    "mp_server_stats_get": (C.c_int, [P, C.POINTER(ServerStats)]),
+    "mp_server_destroy": (C.c_int, [P]),
+}
+EXPORTED = sorted(_SIGS)
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MPError("native library %s is not built (run isaac_sim_ros2_humble_amd/build.py)" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != MP_OK:
+        msg = lib().mp_last_error()
+        raise MPError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def robot_desc(m):
+    """RobotDesc for a model.RobotModel; the returned tuple keeps the arrays alive."""
+    keep = [np.ascontiguousarray(m.joint_types, np.int32), np.ascontiguousarray(m.joint_origins, np.float64),
+            np.ascontiguousarray(m.joint_axes, np.float64), np.ascontiguousarray(m.parents, np.int32),
+            np.ascontiguousarray(m.active, np.int32), np.ascontiguousarray(m.lower, np.float32),
+            np.ascontiguousarray(m.upper, np.float32), np.ascontiguousarray(m.defaults, np.float32),
+            np.ascontiguousarray(m.sphere_link, np.int32), np.ascontiguousarray(m.sphere_pos, np.float32),
+            np.ascontiguousarray(m.sphere_radius, np.float32), np.ascontiguousarray(m.enabled, np.uint8)]
+    d = RobotDesc(m.name.encode(), m.L, _p(keep[0]), _p(keep[1]), _p(keep[2]), _p(keep[3]), _p(keep[4]), _p(keep[5]),
+                  _p(keep[6]), _p(keep[7]), m.S, _p(keep[8]), _p(keep[9]), _p(keep[10]), _p(keep[11]))
+    return d, keep
+
+
+def precompile(models):
+    descs = [robot_desc(m) for m in models]
+    arr = (C.POINTER(RobotDesc) * len(descs))(*[C.pointer(d[0]) for d in descs])
+    check(lib().mp_precompile(C.cast(arr, C.c_void_p), len(descs)), "mp_precompile")
+
+
+def generate_source(models):
+    descs = [robot_desc(m) for m in models]
+    arr = (C.POINTER(RobotDesc) * len(descs))(*[C.pointer(d[0]) for d in descs])
+    n = lib().mp_generate_source(C.cast(arr, C.c_void_p), len(descs), None, 0)
+    if n == 0:
+        raise MPError("mp_generate_source failed: %s" % lib().mp_last_error().decode())
+    buf = C.create_string_buffer(n + 1)
+    lib().mp_generate_source(C.cast(arr, C.c_void_p), len(descs), buf, n + 1)
+    return buf.value.decode()
+
+
+_initialised = False
+
+
+def init(device=0):
+    global _initialised
+    if not _initialised:
+        check(lib().mp_init(device), "mp_init")
+        _initialised = True
+
+
+class DeviceBuffer:
+    """Device allocation owned by the library (hipMalloc through the C ABI)."""
+
+    def __init__(self, nbytes):
+        init()
+        self.nbytes = int(nbytes)
+        ptr = C.c_void_p()
+        check(lib().mp_device_alloc(C.byref(ptr), max(self.nbytes, 1)), "mp_device_alloc")
+        self.ptr = ptr.value
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        if a.nbytes:
+            check(lib().mp_memcpy_h2d(b.ptr, _p(a), a.nbytes, None), "mp_memcpy_h2d")
+        return b
+
+    def to_array(self, dtype, shape):
+        out = np.empty(shape, dtype)
+        if out.nbytes:
+            check(lib().mp_memcpy_d2h(_p(out), self.ptr, out.nbytes, None), "mp_memcpy_d2h")
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.mp_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+class Robot:
+    def __init__(self, model):
+        init()
+        self.model = model
+        d, self._keep = robot_desc(model)
+        h = C.c_void_p()
+        check(lib().mp_robot_create(C.byref(d), C.byref(h)), "mp_robot_create")
+        self.h = h.value
+        dof, ns, npairs, nl = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().mp_robot_info(self.h, C.byref(dof), C.byref(ns), C.byref(npairs), C.byref(nl)), "mp_robot_info")
+        self.dof, self.S, self.P, self.L = dof.value, ns.value, npairs.value, nl.value
+
+    def source(self):
+        n = lib().mp_robot_source(self.h, None, 0)
+        buf = C.create_string_buffer(n + 1)
+        lib().mp_robot_source(self.h, buf, n + 1)
+        return buf.value.decode()
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.mp_robot_destroy(self.h)
+
+
+class Scene:
+    def __init__(self, scene):
+        init()
+        self.scene = scene
+        self._keep = [scene.spheres, scene.boxes, scene.cylinders, scene.capsules]
+        ns, nb, nc, ncap = scene.counts
+        h = C.c_void_p()
+        check(lib().mp_scene_create(_p(scene.spheres), ns, _p(scene.boxes), nb, _p(scene.cylinders), nc,
+                                    _p(scene.capsules), ncap, C.byref(h)), "mp_scene_create")
+        self.h = h.value
+
+    def update(self, scene):
+        self.scene = scene
+        self._keep = [scene.spheres, scene.boxes, scene.cylinders, scene.capsules]
+        ns, nb, nc, ncap = scene.counts
+        check(lib().mp_scene_update(self.h, _p(scene.spheres), ns, _p(scene.boxes), nb, _p(scene.cylinders), nc,
+                                    _p(scene.capsules), ncap), "mp_scene_update")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.mp_scene_destroy(self.h)
+
+
+def work_flops(reset=True):
+    v = C.c_uint64()
+    check(lib().mp_work_flops(C.byref(v), 1 if reset else 0), "mp_work_flops")
+    return v.value
+
+
+def kernel_ms(family):
+    return lib().mp_last_kernel_ms(family.encode())
+
+
+def sync():
+    check(lib().mp_stream_sync(None), "mp_stream_sync")
+
+
+# ---------------------------------------------------------------------------------------------
+# host-array conveniences (copy in, run, copy out); the timed benchmarks use device buffers
+# ---------------------------------------------------------------------------------------------
+def fk_spheres(robot, q):
+    q = np.ascontiguousarray(q, np.float32).reshape(-1, robot.dof)
+    n = len(q)
+    dq = DeviceBuffer.from_array(q)
+    dc = DeviceBuffer(n * robot.S * 12)
+    check(lib().mp_fk_spheres(robot.h, dq.ptr, n, dc.ptr, None), "mp_fk_spheres")
+    return dc.to_array(np.float32, (n, robot.S, 3))
+
+
+def check_states(robot, scene, q, want_centres=False):
+    q = np.ascontiguousarray(q, np.float32).reshape(-1, robot.dof)
+    n = len(q)
+    dq = DeviceBuffer.from_array(q)
+    dv = DeviceBuffer(n)
+    dc = DeviceBuffer(n * robot.S * 12) if want_centres else None
+    check(lib().mp_check_states(robot.h, scene.h, dq.ptr, n, dv.ptr, dc.ptr if dc else None, None), "mp_check_states")
+    v = dv.to_array(np.uint8, (n,))
+    return (v, dc.to_array(np.float32, (n, robot.S, 3))) if want_centres else v
+
+
+def state_costs(robot, scene, q):
+    q = np.ascontiguousarray(q, np.float32).reshape(-1, robot.dof)
+    n = len(q)
+    dq = DeviceBuffer.from_array(q)
+    dcost = DeviceBuffer(4 * n)
+    dv = DeviceBuffer(n)
+    check(lib().mp_state_costs(robot.h, scene.h, dq.ptr, n, dcost.ptr, dv.ptr, None), "mp_state_costs")
+    return dcost.to_array(np.float32, (n,)), dv.to_array(np.uint8, (n,))
+
+
+def check_edges(robot, scene, q0, q1, resolution):
+    q0 = np.ascontiguousarray(q0, np.float32).reshape(-1, robot.dof)
+    q1 = np.ascontiguousarray(q1, np.float32).reshape(-1, robot.dof)
+    m = len(q0)
+    a, b = DeviceBuffer.from_array(q0), DeviceBuffer.from_array(q1)
+    ok, ln = DeviceBuffer(m), DeviceBuffer(4 * m)
+    check(lib().mp_check_edges(robot.h, scene.h, a.ptr, b.ptr, m, resolution, ok.ptr, ln.ptr, None), "mp_check_edges")
+    return ok.to_array(np.uint8, (m,)), ln.to_array(np.float32, (m,))
+
+
+def nn_argmin(tree, queries, capacity=None):
+    """tree: [n, dof] host array (stored SoA on device); queries [nq, dof]."""
+    tree = np.ascontiguousarray(tree, np.float32)
+    n, dof = tree.shape
+    cap = capacity or max(n, 1)
+    soa = np.zeros((dof, cap), np.float32)
+    soa[:, :n] = tree.T
+    q = np.ascontiguousarray(queries, np.float32).reshape(-1, dof)
+    dt, dq = DeviceBuffer.from_array(soa), DeviceBuffer.from_array(q)
+    di = DeviceBuffer(4 * len(q))
+    check(lib().mp_nn_argmin(dt.ptr, n, cap, dof, dq.ptr, len(q), di.ptr, None), "mp_nn_argmin")
+    return di.to_array(np.int32, (len(q),))
+
+
+def sample(robot, seed, offset, n):
+    dq = DeviceBuffer(4 * n * robot.dof)
+    check(lib().mp_sample(robot.h, seed, offset, n, dq.ptr, None), "mp_sample")
+    return dq.to_array(np.float32, (n, robot.dof))
+
+
+def steer(frm, to, max_distance):
+    frm = np.ascontiguousarray(frm, np.float32)
+    to = np.ascontiguousarray(to, np.float32)
+    n, dof = frm.shape
+    a, b = DeviceBuffer.from_array(frm), DeviceBuffer.from_array(to)
+    check(lib().mp_steer(a.ptr, b.ptr, n, dof, max_distance, None), "mp_steer")
+    return b.to_array(np.float32, (n, dof))
+
+
+def default_params(**kw):
+    p = RRTCParams()
+    lib().mp_rrtc_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _path_result(pth, wp, dof):
+    n = max(pth.num_waypoints, 0) if pth.status == 1 else 0
+    return {"status": PLAN_STATUS.get(pth.status, str(pth.status)), "path": wp[:n].reshape(-1, dof).copy(),
+            "iterations": pth.iterations, "start_tree": pth.start_tree_size, "goal_tree": pth.goal_tree_size,
+            "configs_checked": pth.configs_checked, "latency_ms": pth.latency_ms, "work_flops": pth.work_flops}
+
+
+def rrtc_solve(robot, scene, start, goal, params=None, max_path=4096):
+    params = params or default_params()
+    wp = np.zeros((params.max_path, robot.dof), np.float32)
+    pth = Path(0, 0, _p(wp), params.max_path, 0, 0, 0, 0, 0.0, 0.0)
+    s = np.ascontiguousarray(start, np.float32)
+    g = np.ascontiguousarray(goal, np.float32)
+    check(lib().mp_rrtc_solve(robot.h, scene.h, _p(s), _p(g), C.byref(params), C.byref(pth)), "mp_rrtc_solve")
+    return _path_result(pth, wp, robot.dof)
+
+
+def rrtc_solve_batch(robots, scenes, starts, goals, params=None):
+    params = params or default_params()
+    nq = len(robots)
+    s16 = np.zeros((nq, 16), np.float32)
+    g16 = np.zeros((nq, 16), np.float32)
+    wps = []
+    paths = (Path * nq)()
+    for k in range(nq):
+        d = robots[k].dof
+        s16[k, :d] = starts[k]
+        g16[k, :d] = goals[k]
+        wp = np.zeros((params.max_path, d), np.float32)
+        wps.append(wp)
+        paths[k] = Path(0, 0, _p(wp), params.max_path, 0, 0, 0, 0, 0.0, 0.0)
+    rh = (C.c_void_p * nq)(*[r.h for r in robots])
+    sh = (C.c_void_p * nq)(*[s.h for s in scenes])
+    check(lib().mp_rrtc_solve_batch(nq, C.cast(rh, C.c_void_p), C.cast(sh, C.c_void_p), _p(s16), _p(g16),
+                                    C.byref(params), C.cast(paths, C.c_void_p)), "mp_rrtc_solve_batch")
+    return [_path_result(paths[k], wps[k], robots[k].dof) for k in range(nq)]

@@ -1,0 +1,200 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the checker
+or the timed CPU baseline.  The product (isaac_sim_ros2_humble_amd) never imports this package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        fp = np.ctypeslib.ndpointer
+        L.orc_robot_create.restype = P
+        L.orc_robot_create.argtypes = [C.c_int, P, P, P, P, P, P, P, P, C.c_int, P, P, P, P]
+        L.orc_robot_destroy.argtypes = [P]
+        L.orc_robot_dof.argtypes = [P]
+        L.orc_robot_num_pairs.argtypes = [P]
+        L.orc_robot_pairs.argtypes = [P, P, P, P]
+        L.orc_full_flops.argtypes = [P, P]
+        L.orc_full_flops.restype = C.c_double
+        L.orc_scene_create.restype = P
+        L.orc_scene_create.argtypes = [P, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int]
+        L.orc_scene_destroy.argtypes = [P]
+        L.orc_sincosf.argtypes = [C.c_float, P, P]
+        L.orc_fk64.argtypes = [P, P, P, P]
+        L.orc_fk32.argtypes = [P, P, P]
+        L.orc_check32.argtypes = [P, P, P, C.c_int64, P, P, P, P, C.c_int]
+        L.orc_check_centres32.argtypes = [P, P, P, P, C.c_int64, P]
+        L.orc_edges32.argtypes = [P, P, P, P, C.c_int, C.c_float, P, P]
+        L.orc_nn.argtypes = [P, C.c_int, C.c_int, P]
+        L.orc_validate_path.argtypes = [P, P, P, C.c_int, C.c_float, P]
+        L.orc_sample.argtypes = [P, C.c_uint64, C.c_uint64, P]
+        L.orc_philox4x32.argtypes = [C.c_uint32] * 6 + [P]
+        L.orc_rrtconnect.argtypes = [P, P, P, P, C.c_float, C.c_float, C.c_uint64, C.c_int, C.c_double, P, C.c_int, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class PlanStats(C.Structure):
+    _fields_ = [("solved", C.c_int), ("iterations", C.c_int), ("start_tree", C.c_int), ("goal_tree", C.c_int),
+                ("states_checked", C.c_int64), ("seconds", C.c_double)]
+
+
+class OracleRobot:
+    def __init__(self, m):
+        self.m = m
+        self._keep = [np.ascontiguousarray(m.joint_types, np.int32), np.ascontiguousarray(m.joint_origins, np.float64),
+                      np.ascontiguousarray(m.joint_axes, np.float64), np.ascontiguousarray(m.parents, np.int32),
+                      np.ascontiguousarray(m.active, np.int32), np.ascontiguousarray(m.lower, np.float32),
+                      np.ascontiguousarray(m.upper, np.float32), np.ascontiguousarray(m.defaults, np.float32),
+                      np.ascontiguousarray(m.sphere_link, np.int32), np.ascontiguousarray(m.sphere_pos, np.float32),
+                      np.ascontiguousarray(m.sphere_radius, np.float32), np.ascontiguousarray(m.enabled, np.uint8)]
+        k = self._keep
+        self.h = lib().orc_robot_create(m.L, _p(k[0]), _p(k[1]), _p(k[2]), _p(k[3]), _p(k[4]), _p(k[5]), _p(k[6]),
+                                        _p(k[7]), m.S, _p(k[8]), _p(k[9]), _p(k[10]), _p(k[11]))
+        assert self.h, "orc_robot_create failed"
+        self.dof = lib().orc_robot_dof(self.h)
+        self.S = m.S
+        self.P = lib().orc_robot_num_pairs(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_robot_destroy(self.h)
+
+    def pairs(self):
+        i = np.zeros(self.P, np.int32)
+        j = np.zeros(self.P, np.int32)
+        t = np.zeros(self.P, np.float32)
+        lib().orc_robot_pairs(self.h, _p(i), _p(j), _p(t))
+        return i, j, t
+
+    def fk64(self, q):
+        q = np.ascontiguousarray(q, np.float64).reshape(-1, self.dof)
+        out = np.zeros((len(q), self.S, 3), np.float64)
+        for n in range(len(q)):
+            lib().orc_fk64(self.h, _p(q[n]), _p(out[n]), None)
+        return out
+
+    def link_poses64(self, q):
+        q = np.ascontiguousarray(q, np.float64).reshape(self.dof)
+        c = np.zeros((self.S, 3))
+        T = np.zeros((self.m.L, 4, 4))
+        lib().orc_fk64(self.h, _p(q), _p(c), _p(T))
+        return T
+
+    def fk32(self, q):
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, self.dof)
+        out = np.zeros((len(q), self.S, 3), np.float32)
+        for n in range(len(q)):
+            lib().orc_fk32(self.h, _p(q[n]), _p(out[n]))
+        return out
+
+    def sample(self, seed, idx0, n):
+        out = np.zeros((n, self.dof), np.float32)
+        for k in range(n):
+            lib().orc_sample(self.h, seed, idx0 + k, _p(out[k]))
+        return out
+
+
+class OracleScene:
+    def __init__(self, scene):
+        self.scene = scene
+        self._keep = [scene.spheres, scene.boxes, scene.cylinders, scene.capsules]
+        ns, nb, nc, ncap = scene.counts
+        self.h = lib().orc_scene_create(_p(scene.spheres), ns, _p(scene.boxes), nb, _p(scene.cylinders), nc,
+                                        _p(scene.capsules), ncap)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_scene_destroy(self.h)
+
+
+def check(robot, scene, q, want_cost=False, want_centres=False, want_flops=False, nthreads=1):
+    q = np.ascontiguousarray(q, np.float32).reshape(-1, robot.dof)
+    n = len(q)
+    v = np.zeros(n, np.uint8)
+    cost = np.zeros(n, np.float32) if want_cost else None
+    cen = np.zeros((n, robot.S, 3), np.float32) if want_centres else None
+    fl = np.zeros(n, np.float64) if want_flops else None
+    lib().orc_check32(robot.h, scene.h, _p(q), n, _p(v), _p(cost), _p(cen), _p(fl), nthreads)
+    return v, cost, cen, fl
+
+
+def check_centres(robot, scene, q, centres):
+    q = np.ascontiguousarray(q, np.float32).reshape(-1, robot.dof)
+    c = np.ascontiguousarray(centres, np.float32)
+    v = np.zeros(len(q), np.uint8)
+    lib().orc_check_centres32(robot.h, scene.h, _p(q), _p(c), len(q), _p(v))
+    return v
+
+
+def edges(robot, scene, q0, q1, resolution):
+    q0 = np.ascontiguousarray(q0, np.float32).reshape(-1, robot.dof)
+    q1 = np.ascontiguousarray(q1, np.float32).reshape(-1, robot.dof)
+    ok = np.zeros(len(q0), np.uint8)
+    ln = np.zeros(len(q0), np.float32)
+    lib().orc_edges32(robot.h, scene.h, _p(q0), _p(q1), len(q0), resolution, _p(ok), _p(ln))
+    return ok, ln
+
+
+def nn(tree, q):
+    tree = np.ascontiguousarray(tree, np.float32)
+    q = np.ascontiguousarray(q, np.float32)
+    return lib().orc_nn(_p(tree), tree.shape[0], tree.shape[1], _p(q))
+
+
+def validate_path(robot, scene, path, resolution=0.005):
+    path = np.ascontiguousarray(path, np.float32).reshape(-1, robot.dof)
+    bad = C.c_int(0)
+    ok = lib().orc_validate_path(robot.h, scene.h, _p(path), len(path), resolution, C.byref(bad))
+    return bool(ok), bad.value
+
+
+def sincosf(x):
+    s = np.zeros(1, np.float32)
+    c = np.zeros(1, np.float32)
+    lib().orc_sincosf(float(x), _p(s), _p(c))
+    return float(s[0]), float(c[0])
+
+
+def philox(c, k):
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32(c[0], c[1], c[2], c[3], k[0], k[1], _p(out))
+    return out
+
+
+def full_flops(robot, scene):
+    return lib().orc_full_flops(robot.h, scene.h)
+
+
+def rrtconnect(robot, scene, start, goal, range_=0.3, resolution=0.02, seed=0, max_iterations=1_000_000,
+               time_limit=10.0, out_cap=100000):
+    start = np.ascontiguousarray(start, np.float32)
+    goal = np.ascontiguousarray(goal, np.float32)
+    out = np.zeros((out_cap, robot.dof), np.float32)
+    st = PlanStats()
+    k = lib().orc_rrtconnect(robot.h, scene.h, _p(start), _p(goal), range_, resolution, seed, max_iterations,
+                             time_limit, _p(out), out_cap, C.byref(st))
+    return out[:k].copy(), st

@@ -1,0 +1,128 @@
+"""GPU RRT-Connect (plain and batched): statuses, path validity re-checked by the CPU oracle at
+0.005 rad, endpoint identity, determinism per seed in exclusive mode, batch == solo."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import point_robot
+from isaac_sim_ros2_humble_amd import native, workloads
+from isaac_sim_ros2_humble_amd.scene import Scene, box_record
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_check(model, scene, q):
+    return native.check_states(_robot(model), native.Scene(scene), q)
+
+
+_ROBOTS = {}
+
+
+def _robot(model):
+    if model.name not in _ROBOTS:
+        native.init(0)
+        _ROBOTS[model.name] = native.Robot(model)
+    return _ROBOTS[model.name]
+
+
+def gpu_edge(model, scene, a, b):
+    ok, _ = native.check_edges(_robot(model), native.Scene(scene), a, b, 0.02)
+    return ok
+
+
+def _assert_valid(model, scene, res, start, goal):
+    assert res["status"] == "solved", res
+    path = res["path"]
+    assert len(path) >= 2
+    assert np.array_equal(path[0], np.asarray(start, np.float32))
+    assert np.array_equal(path[-1], np.asarray(goal, np.float32))
+    ok, bad = oracle.validate_path(oracle.OracleRobot(model), oracle.OracleScene(scene), path, 0.005)
+    return ok
+
+
+def test_point_robot_wall():
+    m = point_robot(0.02, lim=1.0)
+    r = _robot(m)
+    scene = Scene(boxes=box_record([0, 0, 0], np.eye(3), [0.05, 0.6, 0.6])[None])
+    s, g = np.array([-0.5, 0, 0], np.float32), np.array([0.5, 0, 0], np.float32)
+    res = native.rrtc_solve(r, native.Scene(scene), s, g, native.default_params(seed=1))
+    assert _assert_valid(m, scene, res, s, g)
+    # the join gap (< 2 res) is not collision checked (CPRRTC semantics): path still clear here
+    assert res["start_tree"] >= 1 and res["goal_tree"] >= 1
+
+
+def test_fr3_c1_query(fr3):
+    scene = workloads.c1_scene(seed=0)
+    rng = np.random.default_rng(0)
+    pair = workloads.sample_pair(fr3, scene, rng, gpu_check, gpu_edge, want_free_line=False)
+    assert pair is not None
+    s, g = pair
+    res = native.rrtc_solve(_robot(fr3), native.Scene(scene), s, g, native.default_params(seed=0))
+    _assert_valid(fr3, scene, res, s, g)  # status + endpoints; validity rate is asserted over batches
+
+
+def test_invalid_start_reported(fr3):
+    scene = Scene(boxes=box_record([0, 0, 0.3], np.eye(3), [0.5, 0.5, 0.5])[None])  # swallows the arm
+    q = workloads.uniform_configs(fr3, 2, np.random.default_rng(0))
+    res = native.rrtc_solve(_robot(fr3), native.Scene(scene), q[0], q[1])
+    assert res["status"] == "invalid" and len(res["path"]) == 0
+
+
+def test_exhausted_and_tree_full():
+    m = point_robot(0.02, lim=1.0)
+    r = _robot(m)
+    # goal enclosed by a box shell: unreachable
+    walls = [box_record([0.5 + dx, dy, dz], np.eye(3), h) for dx, dy, dz, h in [
+        (-0.15, 0, 0, [0.01, 0.2, 0.2]), (0.15, 0, 0, [0.01, 0.2, 0.2]), (0, -0.15, 0, [0.2, 0.01, 0.2]),
+        (0, 0.15, 0, [0.2, 0.01, 0.2]), (0, 0, -0.15, [0.2, 0.2, 0.01]), (0, 0, 0.15, [0.2, 0.2, 0.01])]]
+    scene = native.Scene(Scene(boxes=np.array(walls)))
+    s, g = np.array([-0.5, 0, 0], np.float32), np.array([0.5, 0, 0], np.float32)
+    res = native.rrtc_solve(r, scene, s, g, native.default_params(max_iterations=300, seed=2))
+    assert res["status"] == "exhausted" and res["iterations"] == 300
+    res = native.rrtc_solve(r, scene, s, g, native.default_params(max_iterations=100000, tree_capacity=64, seed=2))
+    assert res["status"] == "tree_full"
+
+
+def test_exclusive_mode_deterministic_and_batch_equals_solo(fr3, fetch):
+    rng = np.random.default_rng(5)
+    queries = []
+    for k in range(6):
+        m = fr3 if k % 2 == 0 else fetch
+        scene = workloads.random_scene(rng, 20, *workloads.workspace(m))
+        pair = workloads.sample_pair(m, scene, rng, gpu_check, gpu_edge, want_free_line=(k % 3 == 0))
+        assert pair is not None
+        queries.append((m, scene, pair[0], pair[1]))
+    p = native.default_params(exclusive=1, seed=11)
+    solo = [native.rrtc_solve(_robot(m), native.Scene(sc), s, g, native.default_params(exclusive=1, seed=11 + k))
+            for k, (m, sc, s, g) in enumerate(queries)]
+    again = native.rrtc_solve(_robot(queries[0][0]), native.Scene(queries[0][1]), queries[0][2], queries[0][3],
+                              native.default_params(exclusive=1, seed=11))
+    assert np.array_equal(again["path"], solo[0]["path"])
+    batch = native.rrtc_solve_batch([_robot(m) for m, _, _, _ in queries], [native.Scene(sc) for _, sc, _, _ in queries],
+                                    [s for _, _, s, _ in queries], [g for _, _, _, g in queries], p)
+    for k, (m, sc, s, g) in enumerate(queries):
+        assert batch[k]["status"] == solo[k]["status"]
+        assert np.array_equal(batch[k]["path"], solo[k]["path"])
+        if solo[k]["status"] == "solved":
+            _assert_valid(m, sc, solo[k], s, g)
+
+
+def test_batch_shared_mixed_robots(fr3, fetch):
+    rng = np.random.default_rng(9)
+    robots, scenes, starts, goals, models = [], [], [], [], []
+    for k in range(24):
+        m = fr3 if k % 2 == 0 else fetch
+        scene = workloads.random_scene(rng, workloads.log_uniform_count(rng, 5, 60), *workloads.workspace(m))
+        pair = workloads.sample_pair(m, scene, rng, gpu_check, gpu_edge, want_free_line=(k % 2 == 0))
+        if pair is None:
+            continue
+        robots.append(_robot(m)); scenes.append(native.Scene(scene)); starts.append(pair[0]); goals.append(pair[1])
+        models.append((m, scene))
+    res = native.rrtc_solve_batch(robots, scenes, starts, goals, native.default_params(seed=3))
+    solved = [r for r in res if r["status"] == "solved"]
+    assert len(solved) >= 0.8 * len(res)
+    valid = 0
+    for (m, sc), r, s, g in zip(models, res, starts, goals):
+        if r["status"] == "solved":
+            valid += _assert_valid(m, sc, r, s, g)
+    assert valid >= 0.9 * len(solved)
