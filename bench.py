@@ -1,0 +1,256 @@
+#!/usr/bin/env python3
+"""Benchmark of the planning hot path on one MI355X.
+
+Headline: solved plan queries/s of the batched planner on the 256-query mixed FR3/Fetch workload
+(one persistent launch per step), next to the plain one-query-per-launch planner on the same
+queries, the served planner's p50/p99 latency under Poisson arrivals, the 2^20-configuration
+FK + collision microbenchmark, and the CPU RRT-Connect baseline (oracle, on the host's cores).
+
+Timed region: K steps of the batched solve, each bracketed by a device synchronisation; the
+inputs (scenes, robot modules, query descriptors) are resident in HBM before timing starts.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--serve-seconds S] [--rate L]
+"""
+import argparse
+import concurrent.futures as cf
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+from isaac_sim_ros2_humble_amd import native, workloads  # noqa: E402
+from isaac_sim_ros2_humble_amd.model import load_robot  # noqa: E402
+
+METRIC = "solved plan queries/sec + p50/p99 latency, batched/served vs plain, 1 MI355X"
+PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_c4(robots, models, n, seed):
+    def check(m, scene, q):
+        return native.check_states(robots[m.name], native.Scene(scene), q)
+
+    def edge(m, scene, a, b):
+        ok, _ = native.check_edges(robots[m.name], native.Scene(scene), a, b, 0.02)
+        return ok
+    return workloads.c4_batch(models, n, check, edge, seed=seed)
+
+
+def run_batch(rob_list, scenes, starts, goals, params):
+    native.sync()
+    t0 = time.perf_counter()
+    res = native.rrtc_solve_batch(rob_list, scenes, starts, goals, params)
+    native.sync()
+    return res, (time.perf_counter() - t0) * 1e3, native.kernel_ms("plan")
+
+
+def full_flops_per_config(m, counts):
+    """Cost model per configuration, full evaluation: FK + 18 S + 9 P + S (11 ns + 34 nb + 35 nc + 28 ncap)."""
+    fk = 0
+    for i in range(1, m.L):
+        t = int(m.joint_types[i])
+        fk += 130 if t == 1 else 87 if t == 2 else 63
+    S = m.S
+    P = 0
+    for i in range(S):
+        for j in range(i + 1, S):
+            li, lj = m.sphere_link[i], m.sphere_link[j]
+            if li != lj and m.enabled[li, lj]:
+                P += 1
+    ns, nb, nc, ncap = counts
+    return fk + 18 * S + 9 * P + S * (11 * ns + 34 * nb + 35 * nc + 28 * ncap)
+
+
+def bench_c2(robot, model, steps, warmup):
+    scene = native.Scene(workloads.c2_scene(seed=1))
+    q = workloads.c2_configs(model, 1 << 20, seed=1)
+    n = len(q)
+    dq = native.DeviceBuffer.from_array(q)
+    dv = native.DeviceBuffer(n)
+    dc = native.DeviceBuffer(n * model.S * 12)
+    dcost = native.DeviceBuffer(4 * n)
+    L = native.lib()
+    out = {}
+    for mode in ("verdict", "full"):
+        times, flops = [], []
+        for it in range(warmup + steps):
+            native.work_flops(reset=True)
+            if mode == "verdict":
+                native.check(L.mp_check_states(robot.h, scene.h, dq.ptr, n, dv.ptr, dc.ptr, None), "check")
+                fam = "check"
+            else:
+                native.check(L.mp_state_costs(robot.h, scene.h, dq.ptr, n, dcost.ptr, dv.ptr, None), "cost")
+                fam = "cost"
+            native.sync()
+            if it >= warmup:
+                times.append(native.kernel_ms(fam))
+                flops.append(native.work_flops(reset=True))
+        ms = float(np.mean(times))
+        fl = float(np.mean(flops))
+        v = dv.to_array(np.uint8, (n,))
+        out[mode] = {"ms": ms, "configs_per_s": n / (ms / 1e3), "valid_fraction": float(v.mean()),
+                     "executed_flops_per_config": fl / n, "achieved_tflops": fl / (ms / 1e3) / 1e12}
+    # bytes: q 28 B + centres 12 S B + verdict 1 B per configuration
+    bpc = model.dof * 4 + 12 * model.S + 1
+    vms = out["verdict"]["ms"]
+    out["verdict"]["hbm_gbs"] = n * bpc / (vms / 1e3) / 1e9
+    ff = full_flops_per_config(model, scene.scene.counts)
+    out["full"]["model_flops_per_config"] = ff
+    out["full"]["model_tflops"] = n * ff / (out["full"]["ms"] / 1e3) / 1e12
+    return out
+
+
+def cpu_baseline(models, queries, budget_s, threads):
+    import oracle  # the CPU baseline leg: the oracle's RRT-Connect (OMPL semantics) on the host
+    orc = {m.name: oracle.OracleRobot(m) for m in models}
+    per_query_limit = 2.0
+    sample = queries[:max(1, min(len(queries), int(budget_s * threads / per_query_limit)))]
+
+    def one(q):
+        k, scene, s, g, _ = q
+        m = models[k]
+        path, st = oracle.rrtconnect(orc[m.name], oracle.OracleScene(scene), s, g, range_=0.3, resolution=0.02,
+                                     seed=0, time_limit=per_query_limit)
+        return st.solved == 1
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        solved = sum(ex.map(one, sample))
+    dt = time.perf_counter() - t0
+    return {"value": solved / dt, "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": "first %d of the %d batched queries, RRT-Connect range 0.3 rad, resolution 0.02 rad, "
+                      "%.0f s limit per query, %d solved in %.1f s" % (len(sample), len(queries), per_query_limit,
+                                                                      solved, dt)}
+
+
+def serve(robots, rob_names, queries, rate, seconds, exclusive=False):
+    srv = native.Server([robots[n] for n in rob_names], tree_memory_bytes=16 << 30, tree_capacity=65536,
+                        max_active=1024, use_graph=True,
+                        params=native.default_params(exclusive=1 if exclusive else 0))
+    arrivals = workloads.poisson_arrivals(rate, seconds, seed=7)
+    scenes = [native.Scene(q[1]) for q in queries]
+    rng = np.random.default_rng(3)
+    tickets = []
+    t0 = time.perf_counter()
+    for k, ta in enumerate(arrivals):
+        while time.perf_counter() - t0 < ta:
+            time.sleep(0.0002)
+        # scene reuse: with probability 0.8 a resident scene of an earlier query, else the next one
+        j = int(rng.integers(len(queries))) if rng.random() < 0.8 else k % len(queries)
+        mi, _, s, g, _ = queries[j]
+        tickets.append((srv.submit(mi, scenes[j], s, g, seed=k), len(s)))
+    srv.wait_idle(120.0)
+    st = srv.stats()
+    statuses = {}
+    for t, dof in tickets:
+        r = srv.poll(t, dof)
+        statuses[r["status"]] = statuses.get(r["status"], 0) + 1
+    srv.close()
+    return {"rate_qps": rate, "seconds": seconds, "submitted": st["submitted"], "completed": st["completed"],
+            "solved": st["solved"], "throughput_qps": st["throughput_qps"], "p50_ms": st["p50_ms"],
+            "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"], "steps": st["steps"], "statuses": statuses}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--queries", type=int, default=256)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--rate", type=float, nargs="*", default=[1000.0])
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--skip", nargs="*", default=[], help="parts to skip: plain serve c2 cpu")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only: queries shard across GPUs with no exchange, so the "
+              "single-GPU number is the per-GPU share (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    native.init(0)
+    models = [load_robot("fr3"), load_robot("fetch")]
+    robots = {m.name: native.Robot(m) for m in models}
+    log("robots ready")
+    queries = build_c4(robots, models, a.queries, seed=0)
+    log("c4 workload: %d queries" % len(queries))
+    rob_list = [robots[models[k].name] for k, _, _, _, _ in queries]
+    scenes = [native.Scene(sc) for _, sc, _, _, _ in queries]
+    starts = [s for _, _, s, _, _ in queries]
+    goals = [g for _, _, _, g, _ in queries]
+    params = native.default_params(seed=0)
+
+    # batched: warmup, then exactly K timed steps
+    for _ in range(a.warmup):
+        run_batch(rob_list, scenes, starts, goals, params)
+    step_ms, kern_ms, solved, flops, checked = [], [], [], [], []
+    for k in range(a.steps):
+        params.seed = 1000 * (k + 1)
+        res, ms, kms = run_batch(rob_list, scenes, starts, goals, params)
+        step_ms.append(ms)
+        kern_ms.append(kms)
+        solved.append(sum(r["status"] == "solved" for r in res))
+        flops.append(sum(r["work_flops"] for r in res))
+        checked.append(sum(r["configs_checked"] for r in res))
+        log("batch step %d: %.1f ms (kernel %.1f ms), solved %d/%d" % (k, ms, kms, solved[-1], len(res)))
+    ms_per_step = float(np.mean(step_ms))
+    batched_qps = float(np.sum(solved)) / (np.sum(step_ms) / 1e3)
+    plan_kernel_ms = float(np.mean(kern_ms))
+    achieved = float(np.mean(flops)) / (plan_kernel_ms / 1e3) / 1e12
+
+    out = {"metric": METRIC, "value": batched_qps, "unit": "queries/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "batched planning: %d independent RRT-Connect queries, FR3 7-DOF / Fetch 8-DOF "
+                                  "mixed, 5-200 obstacles (box/cylinder/sphere), half straight-line-free, tree "
+                                  "capacity 65536, resolution 0.02, one persistent launch per step" % len(queries),
+                      "queries_per_step": len(queries), "tree_capacity": 65536, "resolution": 0.02},
+           "batched": {"solved_per_step": solved, "success_rate": float(np.mean(solved)) / len(queries),
+                       "plan_kernel_ms": plan_kernel_ms, "configs_checked_per_step": float(np.mean(checked))},
+           "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                        "kernel": "mp_plan (batched)", "flops_per_launch": float(np.mean(flops))}}
+
+    if "plain" not in a.skip:
+        t0 = time.perf_counter()
+        nsol, lat = 0, []
+        for k in range(len(queries)):
+            r = native.rrtc_solve(rob_list[k], scenes[k], starts[k], goals[k], native.default_params(seed=k))
+            nsol += r["status"] == "solved"
+            lat.append(r["latency_ms"])
+        dt = time.perf_counter() - t0
+        plain_qps = nsol / dt
+        out["plain"] = {"qps": plain_qps, "solved": nsol, "p50_ms": float(np.percentile(lat, 50)),
+                        "p99_ms": float(np.percentile(lat, 99)), "seconds": dt}
+        out["batched_vs_plain"] = batched_qps / plain_qps if plain_qps > 0 else None
+        log("plain: %.1f q/s" % plain_qps)
+
+    if "serve" not in a.skip:
+        out["served"] = [serve(robots, [m.name for m in models], queries, rate, a.serve_seconds) for rate in a.rate]
+        log("served:", out["served"])
+
+    if "c2" not in a.skip:
+        out["c2"] = bench_c2(robots["fr3"], models[0], max(a.steps, 3), 1)
+        c2v = out["c2"]["verdict"]
+        out["roofline_c2"] = {"bound": "valu", "kernel": "mp_check_0 (2^20 FR3 configs, 64 obstacles)",
+                              "achieved": c2v["achieved_tflops"], "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                              "frac": c2v["achieved_tflops"] / PEAK_FP32_TFLOPS, "hbm_gbs": c2v["hbm_gbs"],
+                              "hbm_frac": c2v["hbm_gbs"] / PEAK_HBM_GBS}
+        log("c2:", out["c2"])
+
+    if "cpu" not in a.skip:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(models, queries, a.cpu_seconds, threads)
+        log("cpu:", out["cpu_baseline"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -52,6 +52,23 @@ def _precompile_one(names):
     return names
 
 
+CPP = os.path.join(HERE, "cpp")
+CPP_LIB = os.path.join(HERE, "libcudampl_amd.so")
+CPP_TEST_SRC = os.path.join(REPO, "tests", "cpp", "test_cpp_api.cpp")
+CPP_TEST_BIN = os.path.join(REPO, "tests", "cpp", "test_cpp_api")
+
+
+def compile_cpp_api():
+    """C++ planner API (CPRRTC::RobotSolver, CUDAMPLib classes) over the C ABI, plus its test."""
+    inc = ["-I" + os.path.join(REPO, "include"), "-I" + os.path.join(CPP, "include")]
+    srcs = [os.path.join(CPP, "src", f) for f in ("RobotSolver.cpp", "cudampl.cpp")]
+    subprocess.check_call(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wno-unused-result"] + inc + srcs +
+                          ["-o", CPP_LIB, "-L" + HERE, "-Wl,-rpath," + HERE, "-lmpamd"])
+    if os.path.exists(CPP_TEST_SRC):
+        subprocess.check_call(["g++", "-std=c++17", "-O2", "-Wall"] + inc + [CPP_TEST_SRC, "-o", CPP_TEST_BIN, "-L" + HERE,
+                              "-Wl,-rpath," + HERE, "-lcudampl_amd", "-lmpamd"])
+
+
 def precompile_bundled():
     """hipRTC-compile the modules for the bundled robots into the on-disk cache (one process per
     robot set, in parallel)."""
@@ -65,6 +82,7 @@ def precompile_bundled():
 
 def build(precompile=True, verbose=False):
     compile_lib(verbose)
+    compile_cpp_api()
     if precompile:
         precompile_bundled()
 

@@ -1,0 +1,3 @@
+// Reference include path <constraints/EnvConstraintSphere.h>: forwards to the CUDAMPLib API over the MI355X C ABI.
+#pragma once
+#include "../CUDAMPLib/mp_cudampl.h"

@@ -1,0 +1,3 @@
+// Reference include path <spaces/SingleArmSpace.h>: forwards to the CUDAMPLib API over the MI355X C ABI.
+#pragma once
+#include "../CUDAMPLib/mp_cudampl.h"

@@ -1,10 +1,347 @@
-// Continuous-batching planning server (transitional stubs).
+// Continuous-batching planning server.
+//
+// One worker thread owns the device.  Each step it admits queued queries into free tree slots
+// (a slot = the fixed tree/path memory of one query, so admission is by free tree memory),
+// launches the admission kernel for the new queries and one persistent planning launch with a
+// ticket budget (extend-connect iterations) shared by every active query, copies the compact
+// query states back, and retires the queries that finished.  Queries that arrive while others
+// are on the device join at the next step; finished ones free their slot immediately.  The
+// step (control-block copy, admission kernel, planning kernel, state copy) can be captured once
+// in a hipGraph and replayed.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <thread>
+#include <unordered_map>
+
 #include "mp_plan.h"
-extern "C" {
-int mp_server_create(const mp_server_params*, mp_server**) { return MP_E_STATE; }
-int mp_server_submit(mp_server*, int, const mp_scene*, const float*, const float*, uint64_t, int64_t*) { return MP_E_STATE; }
-int mp_server_poll(mp_server*, int64_t, mp_path*) { return MP_E_STATE; }
-int mp_server_wait_idle(mp_server*, double) { return MP_E_STATE; }
-int mp_server_stats_get(mp_server*, mp_server_stats*) { return MP_E_STATE; }
-int mp_server_destroy(mp_server*) { return MP_E_STATE; }
+
+using mp::Error;
+
+namespace {
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
+
+struct Pending {
+    int64_t ticket;
+    int robot;
+    const mp_scene* scene;
+    float start[16];
+    float goal[16];
+    uint64_t seed;
+    double t_submit;
+};
+
+struct Done {
+    mp_path info;
+    std::vector<float> waypoints;
+};
+
+struct ActiveQ {
+    int64_t ticket;
+    double t_submit;
+    int dof;
+};
+
+}  // namespace
+
+struct mp_server {
+    mp_server_params prm;
+    std::vector<mp_robot*> robots;
+    std::shared_ptr<mp::Module> mod;
+    std::unique_ptr<mp::SlotPool> pool;
+    int nslots = 0, waves = 0, budget = 0;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_def = nullptr;
+
+    std::thread worker;
+    std::mutex mu;
+    std::condition_variable cv, idle_cv;
+    bool stop = false;
+    std::string error;
+
+    int64_t next_ticket = 1;
+    std::deque<Pending> queue;
+    std::map<int, ActiveQ> active;  // slot -> query
+    std::vector<int> free_slots;
+    std::unordered_map<int64_t, Done> done;
+
+    // stats
+    int64_t submitted = 0, completed = 0, solved = 0, steps = 0;
+    std::vector<double> latencies;
+    double t_first = -1.0, t_last = 0.0;
+
+    void run();
+    void step(const std::vector<int>& list, int n_new);
+};
+
+void mp_server::step(const std::vector<int>& list, int n_new) {
+    int* h = pool->hpinned;
+    h[0] = 0;
+    h[1] = (int)list.size();
+    h[2] = (int)list.size();
+    h[3] = budget;
+    h[4] = (int)list.size() - n_new;
+    h[5] = n_new;
+    h[6] = prm.params.exclusive ? 1 : 0;
+    for (size_t k = 0; k < list.size(); k++) h[16 + k] = list[k];
+    for (int k = (int)list.size() - n_new; k < (int)list.size(); k++) {
+        int slot = list[k];
+        MP_HIP(hipMemcpyAsync(pool->dq + slot, &pool->hq[slot], sizeof(mp::QueryDevHost), hipMemcpyHostToDevice, stream));
+    }
+    if (prm.use_graph) {
+        if (!graph) {
+            mp::PlanCtlHost ctl = pool->ctl(prm.params.resolution, prm.params.max_step);
+            void* args[] = {&ctl};
+            MP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            MP_HIP(hipMemcpyAsync(pool->dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, stream));
+            MP_HIP(hipMemcpyAsync(pool->dactive, h + 16, (size_t)nslots * sizeof(int), hipMemcpyHostToDevice, stream));
+            MP_HIP(hipModuleLaunchKernel(mod->fn("mp_plan_init"), nslots, 1, 1, 64, 1, 1, 0, stream, args, nullptr));
+            MP_HIP(hipModuleLaunchKernel(mod->fn("mp_plan"), waves, 1, 1, 64, 1, 1, 0, stream, args, nullptr));
+            MP_HIP(hipMemcpyAsync(pool->hst_pinned, pool->dst, (size_t)nslots * sizeof(mp::QueryStateHost),
+                                  hipMemcpyDeviceToHost, stream));
+            MP_HIP(hipStreamEndCapture(stream, &graph_def));
+            MP_HIP(hipGraphInstantiate(&graph, graph_def, nullptr, nullptr, 0));
+        }
+        MP_HIP(hipGraphLaunch(graph, stream));
+    } else {
+        mp::PlanCtlHost ctl = pool->ctl(prm.params.resolution, prm.params.max_step);
+        void* args[] = {&ctl};
+        MP_HIP(hipMemcpyAsync(pool->dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, stream));
+        MP_HIP(hipMemcpyAsync(pool->dactive, h + 16, list.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+        if (n_new > 0) mp::launch(mod->fn("mp_plan_init"), (unsigned)n_new, 64, stream, args, "plan_init");
+        mp::launch(mod->fn("mp_plan"), (unsigned)waves, 64, stream, args, "plan");
+        MP_HIP(hipMemcpyAsync(pool->hst_pinned, pool->dst, (size_t)nslots * sizeof(mp::QueryStateHost),
+                              hipMemcpyDeviceToHost, stream));
+    }
+    MP_HIP(hipStreamSynchronize(stream));
+}
+
+void mp_server::run() {
+    try {
+        std::vector<int> list;
+        std::vector<int> fresh;
+        while (true) {
+            {
+                std::unique_lock<std::mutex> g(mu);
+                cv.wait(g, [&] { return stop || !queue.empty() || !active.empty(); });
+                if (stop) break;
+                fresh.clear();
+                while (!queue.empty() && !free_slots.empty()) {
+                    Pending p = queue.front();
+                    queue.pop_front();
+                    int slot = free_slots.back();
+                    free_slots.pop_back();
+                    mp_robot* r = robots[p.robot];
+                    pool->fill_query(slot, p.robot, r->model.dof, p.scene, p.start, p.goal, p.seed, prm.params.max_iterations);
+                    active[slot] = ActiveQ{p.ticket, p.t_submit, r->model.dof};
+                    fresh.push_back(slot);
+                }
+                list.clear();
+                for (auto& kv : active)
+                    if (std::find(fresh.begin(), fresh.end(), kv.first) == fresh.end()) list.push_back(kv.first);
+                for (int s : fresh) list.push_back(s);
+            }
+            if (list.empty()) continue;
+            step(list, (int)fresh.size());
+            steps++;
+            // retire finished queries
+            std::vector<std::pair<int64_t, Done>> fin;
+            std::vector<int> freed;
+            for (int slot : list) {
+                const mp::QueryStateHost& st = pool->hst_pinned[slot];
+                if (st.status == MP_PLAN_RUNNING || st.status < 0) continue;
+                ActiveQ aq;
+                {
+                    std::lock_guard<std::mutex> g(mu);
+                    aq = active[slot];
+                }
+                Done d;
+                std::memset(&d.info, 0, sizeof(d.info));
+                d.waypoints.resize((size_t)pool->max_path * aq.dof);
+                d.info.waypoints = d.waypoints.data();
+                d.info.capacity = pool->max_path;
+                mp::harvest(*pool, slot, aq.dof, st, &d.info, stream);
+                fin.emplace_back(aq.ticket, std::move(d));
+                freed.push_back(slot);
+            }
+            if (!fin.empty()) MP_HIP(hipStreamSynchronize(stream));
+            double t = now_ms();
+            {
+                std::lock_guard<std::mutex> g(mu);
+                for (size_t k = 0; k < fin.size(); k++) {
+                    int slot = freed[k];
+                    double lat = t - active[slot].t_submit;
+                    fin[k].second.info.latency_ms = lat;
+                    fin[k].second.waypoints.resize((size_t)std::max(fin[k].second.info.num_waypoints, 0) * active[slot].dof);
+                    fin[k].second.info.waypoints = nullptr;
+                    latencies.push_back(lat);
+                    completed++;
+                    if (fin[k].second.info.status == MP_PLAN_SOLVED) solved++;
+                    t_last = t;
+                    done.emplace(fin[k].first, std::move(fin[k].second));
+                    active.erase(slot);
+                    free_slots.push_back(slot);
+                }
+            }
+            idle_cv.notify_all();
+        }
+    } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> g(mu);
+        error = e.what();
+        stop = true;
+    }
+    idle_cv.notify_all();
+}
+
+extern "C" {
+
+int mp_server_create(const mp_server_params* params, mp_server** out) {
+    return mp::guard_call([&] {
+        if (mp::lib_stream() == nullptr) throw Error(MP_E_STATE, "mp_init() has not been called");
+        if (!params || !out || params->num_robots < 1 || !params->robots) throw Error(MP_E_INVALID, "bad server params");
+        if (params->tree_capacity < 2 || params->max_active < 1) throw Error(MP_E_INVALID, "bad server capacities");
+        const mp_rrtc_params& p = params->params;
+        if (!(p.resolution > 0.0f) || p.max_step < 1 || p.max_step > 64 || p.max_iterations < 1 || p.max_path < 2)
+            throw Error(MP_E_INVALID, "bad planner params");
+        auto s = std::make_unique<mp_server>();
+        s->prm = *params;
+        s->robots.assign(params->robots, params->robots + params->num_robots);
+        int dof_max = 1;
+        std::vector<const mp::RobotModel*> ms;
+        for (auto* r : s->robots) {
+            if (!r) throw Error(MP_E_INVALID, "null robot");
+            ms.push_back(&r->model);
+            dof_max = std::max(dof_max, r->model.dof);
+        }
+        s->mod = ms.size() == 1 ? s->robots[0]->module : mp::get_module(ms);
+        size_t per = mp::slot_bytes(params->tree_capacity, dof_max, p.max_path);
+        size_t fit = params->tree_memory_bytes / per;
+        s->nslots = (int)std::min<size_t>((size_t)params->max_active, fit);
+        if (s->nslots < 1) throw Error(MP_E_CAPACITY, "tree memory too small for one query");
+        s->pool = std::make_unique<mp::SlotPool>(s->nslots, params->tree_capacity, dof_max, p.max_path);
+        s->waves = params->waves > 0 ? params->waves : mp::default_waves(s->mod->fn("mp_plan"));
+        if (p.exclusive) s->waves = s->nslots;
+        s->budget = params->step_iterations > 0 ? params->step_iterations : 2 * s->waves;
+        MP_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        for (int k = s->nslots - 1; k >= 0; k--) s->free_slots.push_back(k);
+        mp_server* raw = s.release();
+        raw->worker = std::thread([raw] { raw->run(); });
+        *out = raw;
+    });
+}
+
+int mp_server_submit(mp_server* s, int robot_index, const mp_scene* scene, const float* start, const float* goal,
+                     uint64_t seed, int64_t* ticket) {
+    return mp::guard_call([&] {
+        if (!s || !scene || !start || !goal || !ticket) throw Error(MP_E_INVALID, "bad submit arguments");
+        if (robot_index < 0 || robot_index >= (int)s->robots.size()) throw Error(MP_E_INVALID, "robot index out of range");
+        Pending p;
+        std::memset(&p, 0, sizeof(p));
+        p.robot = robot_index;
+        p.scene = scene;
+        int dof = s->robots[robot_index]->model.dof;
+        std::memcpy(p.start, start, dof * sizeof(float));
+        std::memcpy(p.goal, goal, dof * sizeof(float));
+        p.seed = seed;
+        p.t_submit = now_ms();
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            if (!s->error.empty()) throw Error(MP_E_STATE, "server stopped: " + s->error);
+            p.ticket = s->next_ticket++;
+            s->queue.push_back(p);
+            s->submitted++;
+            if (s->t_first < 0) s->t_first = p.t_submit;
+        }
+        s->cv.notify_one();
+        *ticket = p.ticket;
+    });
+}
+
+int mp_server_poll(mp_server* s, int64_t ticket, mp_path* out) {
+    int ready = 0;
+    int rc = mp::guard_call([&] {
+        if (!s || !out) throw Error(MP_E_INVALID, "bad poll arguments");
+        std::lock_guard<std::mutex> g(s->mu);
+        if (!s->error.empty()) throw Error(MP_E_STATE, "server stopped: " + s->error);
+        auto it = s->done.find(ticket);
+        if (it == s->done.end()) return;
+        const Done& d = it->second;
+        float* wp = out->waypoints;
+        int cap = out->capacity;
+        *out = d.info;
+        out->waypoints = wp;
+        out->capacity = cap;
+        int dof = d.info.num_waypoints > 0 ? (int)(d.waypoints.size() / d.info.num_waypoints) : 0;
+        int n = std::min(d.info.num_waypoints, cap);
+        if (wp && n > 0) std::memcpy(wp, d.waypoints.data(), (size_t)n * dof * sizeof(float));
+        out->num_waypoints = d.info.status == MP_PLAN_SOLVED ? n : d.info.num_waypoints;
+        s->done.erase(it);
+        ready = 1;
+    });
+    return rc == MP_OK ? ready : rc;
+}
+
+int mp_server_wait_idle(mp_server* s, double timeout_s) {
+    return mp::guard_call([&] {
+        if (!s) throw Error(MP_E_INVALID, "null server");
+        std::unique_lock<std::mutex> g(s->mu);
+        bool ok = s->idle_cv.wait_for(g, std::chrono::duration<double>(timeout_s), [&] {
+            return !s->error.empty() || (s->queue.empty() && s->active.empty());
+        });
+        if (!s->error.empty()) throw Error(MP_E_STATE, "server stopped: " + s->error);
+        if (!ok) throw Error(MP_E_STATE, "timeout waiting for the server to drain");
+    });
+}
+
+int mp_server_stats_get(mp_server* s, mp_server_stats* out) {
+    return mp::guard_call([&] {
+        if (!s || !out) throw Error(MP_E_INVALID, "bad arguments");
+        std::lock_guard<std::mutex> g(s->mu);
+        std::memset(out, 0, sizeof(*out));
+        out->submitted = s->submitted;
+        out->completed = s->completed;
+        out->solved = s->solved;
+        out->active = (int)s->active.size();
+        out->queued = (int)s->queue.size();
+        out->steps = s->steps;
+        if (!s->latencies.empty()) {
+            std::vector<double> v = s->latencies;
+            std::sort(v.begin(), v.end());
+            auto pct = [&](double q) { return v[std::min(v.size() - 1, (size_t)(q * (v.size() - 1) + 0.5))]; };
+            out->p50_ms = pct(0.50);
+            out->p99_ms = pct(0.99);
+            double sum = 0;
+            for (double x : v) sum += x;
+            out->mean_ms = sum / v.size();
+            double span = (s->t_last - s->t_first) / 1000.0;
+            out->throughput_qps = span > 0 ? s->completed / span : 0.0;
+        }
+    });
+}
+
+int mp_server_destroy(mp_server* s) {
+    return mp::guard_call([&] {
+        if (!s) return;
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->stop = true;
+        }
+        s->cv.notify_all();
+        if (s->worker.joinable()) s->worker.join();
+        if (s->stream) hipStreamSynchronize(s->stream);
+        if (s->graph) hipGraphExecDestroy(s->graph);
+        if (s->graph_def) hipGraphDestroy(s->graph_def);
+        s->pool.reset();
+        if (s->stream) hipStreamDestroy(s->stream);
+        delete s;
+    });
+}
+
+}  // extern "C"

@@ -372,3 +372,55 @@ def rrtc_solve_batch(robots, scenes, starts, goals, params=None):
     check(lib().mp_rrtc_solve_batch(nq, C.cast(rh, C.c_void_p), C.cast(sh, C.c_void_p), _p(s16), _p(g16),
                                     C.byref(params), C.cast(paths, C.c_void_p)), "mp_rrtc_solve_batch")
     return [_path_result(paths[k], wps[k], robots[k].dof) for k in range(nq)]
+
+
+class Server:
+    """Continuous-batching planning server (mp_server_*)."""
+
+    def __init__(self, robots, tree_memory_bytes=8 << 30, tree_capacity=65536, max_active=512, step_iterations=0,
+                 waves=0, use_graph=False, params=None):
+        init()
+        self.robots = list(robots)
+        self._rh = (C.c_void_p * len(self.robots))(*[r.h for r in self.robots])
+        p = params or default_params(tree_capacity=tree_capacity)
+        p.tree_capacity = tree_capacity
+        sp = ServerParams(len(self.robots), C.cast(self._rh, C.c_void_p), tree_memory_bytes, tree_capacity, max_active,
+                          step_iterations, waves, 1 if use_graph else 0, p)
+        h = C.c_void_p()
+        check(lib().mp_server_create(C.byref(sp), C.byref(h)), "mp_server_create")
+        self.h = h.value
+        self.max_path = p.max_path
+
+    def submit(self, robot_index, scene, start, goal, seed=0):
+        t = C.c_int64()
+        s = np.ascontiguousarray(start, np.float32)
+        g = np.ascontiguousarray(goal, np.float32)
+        check(lib().mp_server_submit(self.h, robot_index, scene.h, _p(s), _p(g), seed, C.byref(t)), "mp_server_submit")
+        return t.value
+
+    def poll(self, ticket, dof):
+        wp = np.zeros((self.max_path, dof), np.float32)
+        pth = Path(0, 0, _p(wp), self.max_path, 0, 0, 0, 0, 0.0, 0.0)
+        rc = lib().mp_server_poll(self.h, ticket, C.byref(pth))
+        if rc < 0:
+            check(rc, "mp_server_poll")
+        return _path_result(pth, wp, dof) if rc == 1 else None
+
+    def wait_idle(self, timeout_s=60.0):
+        check(lib().mp_server_wait_idle(self.h, timeout_s), "mp_server_wait_idle")
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().mp_server_stats_get(self.h, C.byref(st)), "mp_server_stats_get")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            check(_lib.mp_server_destroy(self.h), "mp_server_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

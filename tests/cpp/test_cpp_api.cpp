@@ -1,0 +1,138 @@
+// Exercises the C++ planner API (CPRRTC::RobotSolver and the CUDAMPLib classes) the way the
+// reference's harnesses do (src/foliation_planner/src/cuda_test.cpp TEST_CPRRTC / TEST_Planner).
+// Input: a robot dump written by tests/test_gpu_cpp_api.py ("key v0 v1 ..." lines).  Output:
+// "key value..." lines that the Python test checks against the CPU oracle.
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include <CPRRTC/RobotSolver.h>
+#include <planners/RRG.h>
+#include <spaces/SingleArmSpace.h>
+#include "mp_cpp_util.h"
+
+using V = std::vector<double>;
+
+static std::map<std::string, V> load(const char* path) {
+    std::map<std::string, V> m;
+    std::ifstream f(path);
+    std::string line;
+    while (std::getline(f, line)) {
+        std::istringstream is(line);
+        std::string k;
+        is >> k;
+        double x;
+        while (is >> x) m[k].push_back(x);
+    }
+    return m;
+}
+
+static void print_path(const char* key, const std::vector<std::vector<float>>& p) {
+    std::printf("%s %zu", key, p.size());
+    for (auto& w : p)
+        for (float x : w) std::printf(" %.9g", x);
+    std::printf("\n");
+}
+
+int main(int argc, char** argv) {
+    if (argc < 2) return 2;
+    auto m = load(argv[1]);
+    const int L = (int)m["joint_types"].size(), S = (int)m["sphere_radius"].size();
+    std::vector<int> jt, par, smap;
+    std::vector<mpx::Isometry3d> poses(L);
+    std::vector<mpx::Vector3d> axes(L), spos3(S);
+    std::vector<std::vector<float>> spos(S);
+    std::vector<float> rad, lo, hi, def;
+    std::vector<bool> act;
+    std::vector<std::string> names(L, "l");
+    std::vector<std::vector<bool>> en(L, std::vector<bool>(L));
+    for (int i = 0; i < L; i++) {
+        jt.push_back((int)m["joint_types"][i]);
+        par.push_back((int)m["parents"][i]);
+        act.push_back(m["active"][i] != 0);
+        lo.push_back((float)m["lower"][i]);
+        hi.push_back((float)m["upper"][i]);
+        def.push_back((float)m["defaults"][i]);
+        for (int r = 0; r < 4; r++)
+            for (int c = 0; c < 4; c++) poses[i].matrix()(r, c) = m["joint_origins"][16 * i + 4 * r + c];
+        for (int a = 0; a < 3; a++) axes[i](a) = m["joint_axes"][3 * i + a];
+        for (int j = 0; j < L; j++) en[i][j] = m["enabled"][i * L + j] != 0;
+    }
+    for (int s = 0; s < S; s++) {
+        smap.push_back((int)m["sphere_link"][s]);
+        rad.push_back((float)m["sphere_radius"][s]);
+        for (int a = 0; a < 3; a++) {
+            spos3[s](a) = m["sphere_pos"][3 * s + a];
+            spos[s].push_back((float)m["sphere_pos"][3 * s + a]);
+        }
+    }
+    size_t dof = 0;
+    for (bool b : act) dof += b;
+    std::vector<float> start(m["start"].begin(), m["start"].end()), goal(m["goal"].begin(), m["goal"].end());
+    const V& bx = m["box"];  // x_min x_max y_min y_max z_min z_max x y z roll pitch yaw
+
+    // --- CPRRTC::RobotSolver (cuda_test.cpp:1902-1931)
+    CPRRTC::RobotSolver solver("robot", dof, jt, poses, axes, par, smap, spos3, rad, act, lo, hi, def, names, en, 0.02f);
+    solver.setEnvObstacleCache(4, 2, 1);
+    std::vector<CPRRTC::Sphere> sph;
+    std::vector<CPRRTC::Cuboid> cub{{(float)bx[0], (float)bx[1], (float)bx[2], (float)bx[3], (float)bx[4], (float)bx[5],
+                                     (float)bx[6], (float)bx[7], (float)bx[8], (float)bx[9], (float)bx[10], (float)bx[11]}};
+    std::vector<CPRRTC::Cylinder> cyl;
+    solver.updateEnvObstacle(sph, cub, cyl);
+    auto path = solver.solve(start, goal);
+    print_path("cprrtc_path", path);
+    std::vector<float> shortv(dof - 1, 0.0f);
+    std::printf("cprrtc_bad_size %zu\n", solver.solve(shortv, goal).size());
+    std::vector<CPRRTC::Cuboid> many(3, cub[0]);
+    solver.updateEnvObstacle(sph, many, cyl);  // over capacity: message, obstacles unchanged
+    std::printf("cprrtc_after_overflow %d\n", solver.solve(start, goal).empty() ? 0 : 1);
+
+    // --- CUDAMPLib (cuda_test.cpp TEST_Planner / evaluate_cudampl.cpp)
+    float Rm[9];
+    mpcpp::rpy_to_matrix((float)bx[9], (float)bx[10], (float)bx[11], Rm);
+    auto env = std::make_shared<CUDAMPLib::EnvConstraintCuboid>(
+        "obstacle", std::vector<std::vector<float>>{{(float)bx[6], (float)bx[7], (float)bx[8]}},
+        std::vector<std::vector<float>>{std::vector<float>(Rm, Rm + 9)},
+        std::vector<std::vector<float>>{{(float)bx[1], (float)bx[3], (float)bx[5]}},
+        std::vector<std::vector<float>>{{(float)bx[0], (float)bx[2], (float)bx[4]}});
+    auto self = std::make_shared<CUDAMPLib::SelfCollisionConstraint>("self", smap, rad, en);
+    auto bound = std::make_shared<CUDAMPLib::BoundaryConstraint>("bound", lo, hi, act);
+    std::vector<CUDAMPLib::BaseConstraintPtr> cons{env, self, bound};
+    auto space = std::make_shared<CUDAMPLib::SingleArmSpace>(dof, cons, jt, poses, axes, par, smap, spos, rad, act, lo, hi,
+                                                             def, names, 0.02f);
+    auto states = space->createStatesFromVector({start, goal});
+    std::vector<bool> feas;
+    space->checkStates(states, feas);
+    std::printf("space_feasible %d %d\n", (int)feas[0], (int)feas[1]);
+    std::vector<bool> mf;
+    std::vector<float> mc;
+    space->checkMotions(space->createStatesFromVector({start}), space->createStatesFromVector({goal}), mf, mc);
+    std::printf("space_motion %d %.9g\n", (int)mf[0], mc[0]);
+    auto samples = space->sample(64);
+    std::vector<bool> sf;
+    space->checkStates(samples, sf);
+    int nf = 0;
+    for (bool b : sf) nf += b;
+    std::printf("space_samples %d %d\n", samples->getNumOfStates(), nf);
+
+    auto task = std::make_shared<CUDAMPLib::SingleArmTask>(std::vector<std::vector<float>>{start},
+                                                           std::vector<std::vector<float>>{goal});
+    auto planner = std::make_shared<CUDAMPLib::RRG>(space);
+    planner->setMotionTask(task, false);
+    planner->solve(std::make_shared<CUDAMPLib::TimeoutTermination>(10.0f));
+    std::printf("rrg_solved %d %s\n", (int)task->hasSolution(), task->getFailureReason().c_str());
+    print_path("rrg_path", task->getSolution());
+
+    auto task0 = std::make_shared<CUDAMPLib::SingleArmTask>(std::vector<std::vector<float>>{start},
+                                                            std::vector<std::vector<float>>{goal});
+    auto planner0 = std::make_shared<CUDAMPLib::cRRTC>(space);
+    planner0->setMotionTask(task0, true);
+    planner0->solve(std::make_shared<CUDAMPLib::StepTermination>(0));
+    std::printf("step0 %d %s\n", (int)task0->hasSolution(), task0->getFailureReason().c_str());
+    return 0;
+}

@@ -1,0 +1,60 @@
+"""The C++ planner API (CPRRTC::RobotSolver, CUDAMPLib space / planners) driven like the
+reference's harnesses, its outputs checked with the CPU oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import point_robot
+from isaac_sim_ros2_humble_amd.scene import Scene, box_record
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "test_cpp_api")
+
+
+def dump(m, path, extra):
+    with open(path, "w") as f:
+        for k in ("joint_types", "joint_origins", "joint_axes", "parents", "active", "lower", "upper", "defaults",
+                  "sphere_link", "sphere_pos", "sphere_radius", "enabled"):
+            a = np.asarray(getattr(m, k)).reshape(-1)
+            f.write(k + " " + " ".join(repr(float(x)) for x in a) + "\n")
+        for k, v in extra.items():
+            f.write(k + " " + " ".join(repr(float(x)) for x in v) + "\n")
+
+
+def test_cpp_api(tmp_path):
+    assert os.path.exists(BIN), "C++ API test binary not built"
+    m = point_robot(0.02, lim=1.0)
+    start, goal = [-0.5, 0.0, 0.0], [0.5, 0.0, 0.0]
+    box = [-0.05, 0.05, -0.6, 0.6, -0.6, 0.6, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0]
+    f = tmp_path / "robot.txt"
+    dump(m, str(f), {"start": start, "goal": goal, "box": box})
+    out = subprocess.run([BIN, str(f)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr[-2000:]
+    res = {}
+    for line in out.stdout.splitlines():
+        k, *v = line.split()
+        res[k] = v
+    scene = Scene(boxes=box_record([0, 0, 0], np.eye(3), [0.05, 0.6, 0.6])[None])
+    ro, so = oracle.OracleRobot(m), oracle.OracleScene(scene)
+
+    def path(key):
+        n = int(res[key][0])
+        return np.array([float(x) for x in res[key][1:]], np.float32).reshape(n, 3)
+
+    p = path("cprrtc_path")
+    assert len(p) >= 2 and np.array_equal(p[0], start) and np.array_equal(p[-1], goal)
+    assert oracle.validate_path(ro, so, p, 0.005)[0]
+    assert res["cprrtc_bad_size"] == ["0"]
+    assert res["cprrtc_after_overflow"] == ["1"]
+    assert "exceeds the maximum number of obstacles" in out.stderr
+    assert res["space_feasible"] == ["1", "1"]
+    assert res["space_motion"][0] == "0" and abs(float(res["space_motion"][1]) - 1.0) < 1e-6
+    assert res["space_samples"][0] == "64"
+    assert res["rrg_solved"][0] == "1"
+    q = path("rrg_path")
+    assert np.array_equal(q[0], start) and np.array_equal(q[-1], goal)
+    assert oracle.validate_path(ro, so, q, 0.005)[0]
+    assert res["step0"] == ["0", "MeetTerminationCondition"]

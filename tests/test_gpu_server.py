@@ -1,0 +1,76 @@
+"""Serving (continuous batching): queries admitted while others run, results identical alone and
+in the mix (exclusive mode: one wave per query, deterministic per seed), graph replays equal
+eager steps, statistics sane."""
+import time
+
+import numpy as np
+import pytest
+
+from isaac_sim_ros2_humble_amd import native, workloads
+
+pytestmark = pytest.mark.gpu
+
+_R = {}
+
+
+def robot(m):
+    if m.name not in _R:
+        native.init(0)
+        _R[m.name] = native.Robot(m)
+    return _R[m.name]
+
+
+def make_queries(models, n, seed):
+    def check(m, s, q):
+        return native.check_states(robot(m), native.Scene(s), q)
+
+    def edge(m, s, a, b):
+        return native.check_edges(robot(m), native.Scene(s), a, b, 0.02)[0]
+    return workloads.c4_batch(models, n, check, edge, seed=seed)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_served_equals_solo_exclusive(fr3, fetch, use_graph):
+    models = [fr3, fetch]
+    qs = make_queries(models, 12, seed=21)
+    scenes = [native.Scene(q[1]) for q in qs]
+    solo = [native.rrtc_solve(robot(models[k]), scenes[i], s, g, native.default_params(exclusive=1, seed=100 + i))
+            for i, (k, _, s, g, _) in enumerate(qs)]
+    srv = native.Server([robot(fr3), robot(fetch)], tree_memory_bytes=1 << 30, max_active=5, step_iterations=7,
+                        use_graph=use_graph, params=native.default_params(exclusive=1))
+    tickets = []
+    for i, (k, _, s, g, _) in enumerate(qs):
+        tickets.append(srv.submit(k, scenes[i], s, g, seed=100 + i))
+        time.sleep(0.002)  # arrivals while earlier queries are on the device
+    srv.wait_idle(120.0)
+    for i, (k, _, s, g, _) in enumerate(qs):
+        r = srv.poll(tickets[i], models[k].dof)
+        assert r is not None
+        assert r["status"] == solo[i]["status"]
+        assert np.array_equal(r["path"], solo[i]["path"])
+    st = srv.stats()
+    assert st["completed"] == len(qs) and st["steps"] > 1
+    assert 0 < st["p50_ms"] <= st["p99_ms"]
+    srv.close()
+
+
+def test_served_shared_mode_results_valid(fr3, fetch):
+    import oracle
+    models = [fr3, fetch]
+    qs = make_queries(models, 24, seed=5)
+    scenes = [native.Scene(q[1]) for q in qs]
+    srv = native.Server([robot(fr3), robot(fetch)], tree_memory_bytes=2 << 30, max_active=16, use_graph=True)
+    tickets = [srv.submit(k, scenes[i], s, g, seed=i) for i, (k, _, s, g, _) in enumerate(qs)]
+    srv.wait_idle(120.0)
+    nsol, nvalid = 0, 0
+    for i, (k, sc, s, g, _) in enumerate(qs):
+        r = srv.poll(tickets[i], models[k].dof)
+        if r["status"] == "solved":
+            nsol += 1
+            assert np.array_equal(r["path"][0], s) and np.array_equal(r["path"][-1], g)
+            ok, _ = oracle.validate_path(oracle.OracleRobot(models[k]), oracle.OracleScene(sc), r["path"], 0.005)
+            nvalid += ok
+    assert nsol >= 0.8 * len(qs)
+    assert nvalid >= 0.9 * nsol
+    assert srv.poll(tickets[0], 7) is None  # each result is handed out once
+    srv.close()

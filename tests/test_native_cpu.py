@@ -1,0 +1,64 @@
+"""CPU-side checks of the native library: it loads, exports every entry point the C ABI header
+declares, generates robot source without a device, and reports errors without a GPU."""
+import os
+import re
+
+import numpy as np
+
+from isaac_sim_ros2_humble_amd import native
+from isaac_sim_ros2_humble_amd.model import load_robot
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "mp_abi.h")
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(mp_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_functions_exported():
+    lib = native.lib()
+    names = declared_functions()
+    assert len(names) >= 35
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(native.EXPORTED) >= set(names)
+
+
+def test_generated_source_structure(fr3, fetch):
+    src = native.generate_source([fr3, fetch])
+    assert "struct Robot0" in src and "struct Robot1" in src
+    for k in ("mp_check_0", "mp_check_1", "mp_plan", "mp_plan_init", "mp_edges_1", "mp_nn_generic"):
+        assert k in src
+    # every self pair of the robot appears exactly once in the hit code of its struct
+    r0 = src[src.index("struct Robot0"):src.index("struct Robot1")]
+    hit = r0[r0.index("self_hit"):r0.index("self_cost")]
+    assert hit.count("h |=") == 754
+    r1 = src[src.index("struct Robot1"):src.index("mp_load_cfg") if "mp_load_cfg" in src else None]
+    hit1 = r1[r1.index("self_hit"):r1.index("self_cost")]
+    assert hit1.count("h |=") == 2534
+    # FK evaluates one sin/cos per active joint only
+    fk = r0[r0.index("void fk("):r0.index("write_centres")]
+    assert fk.count("mp_sincosf(") == 7
+
+
+def test_generated_source_is_deterministic_and_robot_specific(fr3):
+    a = native.generate_source([fr3])
+    assert a == native.generate_source([fr3])
+    m2 = load_robot("fr3")
+    m2.sphere_radius = m2.sphere_radius.copy()
+    m2.sphere_radius[5] += np.float32(0.001)
+    assert native.generate_source([m2]) != a  # a different robot never reuses a cached module
+
+
+def test_invalid_robot_rejected():
+    m = load_robot("fr3")
+    m.parents = m.parents.copy()
+    m.parents[3] = 7  # not topologically ordered
+    try:
+        native.generate_source([m])
+    except native.MPError as e:
+        assert "topological" in str(e)
+    else:
+        raise AssertionError("expected an error")
