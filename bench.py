@@ -131,6 +131,29 @@ def cpu_baseline(models, queries, budget_s, threads):
                                                                       solved, dt)}
 
 
+def bench_c3(robot, model, seeds):
+    """Fetch 8-DOF in a 3-shelf unit: start outside, goal between shelves, one query per seed."""
+    scene, shelf = workloads.c3_scene(seed=0)
+    nscene = native.Scene(scene)
+
+    def check(m, sc, q):
+        return native.check_states(robot, nscene, q)
+
+    def fk(m, q):
+        return native.fk_spheres(robot, q)
+    starts, goals = workloads.c3_queries(model, scene, shelf, seeds, check, fk, seed=0)
+    ok, lens, times = 0, [], []
+    for k in range(seeds):
+        r = native.rrtc_solve(robot, nscene, starts[k], goals[k], native.default_params(seed=k))
+        times.append(r["latency_ms"])
+        if r["status"] == "solved":
+            ok += 1
+            lens.append(float(np.linalg.norm(np.diff(r["path"], axis=0), axis=1).sum()))
+    return {"seeds": seeds, "success_rate": ok / seeds, "mean_path_length_rad": float(np.mean(lens)) if lens else None,
+            "mean_ms": float(np.mean(times)), "p50_ms": float(np.percentile(times, 50)),
+            "p99_ms": float(np.percentile(times, 99)), "obstacles": scene.num_obstacles}
+
+
 def serve(robots, rob_names, queries, rate, seconds, exclusive=False):
     srv = native.Server([robots[n] for n in rob_names], tree_memory_bytes=16 << 30, tree_capacity=65536,
                         max_active=1024, use_graph=True,
@@ -168,7 +191,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--rate", type=float, nargs="*", default=[1000.0])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--skip", nargs="*", default=[], help="parts to skip: plain serve c2 cpu")
+    ap.add_argument("--c3-seeds", type=int, default=100)
+    ap.add_argument("--skip", nargs="*", default=[], help="parts to skip: plain serve c2 c3 cpu")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: queries shard across GPUs with no exchange, so the "
@@ -244,6 +268,10 @@ def main():
                               "frac": c2v["achieved_tflops"] / PEAK_FP32_TFLOPS, "hbm_gbs": c2v["hbm_gbs"],
                               "hbm_frac": c2v["hbm_gbs"] / PEAK_HBM_GBS}
         log("c2:", out["c2"])
+
+    if "c3" not in a.skip and a.c3_seeds > 0:
+        out["c3"] = bench_c3(robots["fetch"], models[1], a.c3_seeds)
+        log("c3:", out["c3"])
 
     if "cpu" not in a.skip:
         threads = min(16, os.cpu_count() or 1)

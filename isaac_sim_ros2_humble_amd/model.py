@@ -30,6 +30,7 @@ class RobotModel:
     sphere_pos: np.ndarray       # float32 [S,3]
     sphere_radius: np.ndarray    # float32 [S]
     enabled: np.ndarray          # uint8 [L,L]
+    joint_names: list = None     # name of each link's parent joint ("" for the root)
 
     @property
     def L(self):
@@ -81,4 +82,5 @@ def load_robot(name: str) -> RobotModel:
         sphere_pos=np.asarray(d["sphere_pos"], np.float32).reshape(-1, 3),
         sphere_radius=np.asarray(d["sphere_radius"], np.float32),
         enabled=np.asarray(d["collision_enabled"], np.uint8),
+        joint_names=list(d["joint_names"]),
     )

@@ -16,7 +16,7 @@ def point_robot(radius=0.05, lim=2.0):
         lower=np.array([0, -lim, -lim, -lim], np.float32), upper=np.array([0, lim, lim, lim], np.float32),
         defaults=np.zeros(4, np.float32), sphere_link=np.array([3], np.int32),
         sphere_pos=np.zeros((1, 3), np.float32), sphere_radius=np.array([radius], np.float32),
-        enabled=np.ones((L, L), np.uint8))
+        enabled=np.ones((L, L), np.uint8), joint_names=["", "jx", "jy", "jz"])
 
 
 def point_box_distance(p, pos, R, lo, hi):
