@@ -154,6 +154,25 @@ def bench_c3(robot, model, seeds):
             "p99_ms": float(np.percentile(times, 99)), "obstacles": scene.num_obstacles}
 
 
+def bench_mbm(models):
+    """MotionBenchMaker problems (committed fixtures: 5 per set, 14 sets) in one batched launch."""
+    from isaac_sim_ros2_humble_amd import mbm
+    mm = [mbm.mbm_model(m) for m in models]
+    rob = [native.Robot(m) for m in mm]
+    probs = mbm.load_problems(os.path.join(REPO, "tests", "golden", "mbm"), mm)
+    scenes = [native.Scene(p[2]) for p in probs]
+    res, ms, kms = run_batch([rob[p[1]] for p in probs], scenes, [p[3] for p in probs], [p[4] for p in probs],
+                             native.default_params(seed=0))
+    by = {}
+    for p, r in zip(probs, res):
+        d = by.setdefault(p[0], [0, 0])
+        d[0] += r["status"] == "solved"
+        d[1] += 1
+    return {"problems": len(probs), "solved": sum(v[0] for v in by.values()), "batch_ms": ms,
+            "statuses": {s: sum(r["status"] == s for r in res) for s in set(r["status"] for r in res)},
+            "per_set_solved": {k: "%d/%d" % tuple(v) for k, v in by.items()}}
+
+
 def serve(robots, rob_names, queries, rate, seconds, exclusive=False):
     srv = native.Server([robots[n] for n in rob_names], tree_memory_bytes=16 << 30, tree_capacity=65536,
                         max_active=1024, use_graph=True,
@@ -192,7 +211,7 @@ def main():
     ap.add_argument("--rate", type=float, nargs="*", default=[1000.0])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--c3-seeds", type=int, default=100)
-    ap.add_argument("--skip", nargs="*", default=[], help="parts to skip: plain serve c2 c3 cpu")
+    ap.add_argument("--skip", nargs="*", default=[], help="parts to skip: plain serve c2 c3 mbm cpu")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: queries shard across GPUs with no exchange, so the "
@@ -272,6 +291,10 @@ def main():
     if "c3" not in a.skip and a.c3_seeds > 0:
         out["c3"] = bench_c3(robots["fetch"], models[1], a.c3_seeds)
         log("c3:", out["c3"])
+
+    if "mbm" not in a.skip:
+        out["mbm"] = bench_mbm(models)
+        log("mbm:", out["mbm"])
 
     if "cpu" not in a.skip:
         threads = min(16, os.cpu_count() or 1)

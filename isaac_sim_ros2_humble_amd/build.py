@@ -46,9 +46,10 @@ def compile_lib(verbose=False):
 
 def _precompile_one(names):
     sys.path.insert(0, REPO)
-    from isaac_sim_ros2_humble_amd import native
+    from isaac_sim_ros2_humble_amd import mbm, native
     from isaac_sim_ros2_humble_amd.model import load_robot
-    native.precompile([load_robot(n) for n in names])
+    models = [mbm.mbm_model(load_robot(n[:-4])) if n.endswith("_mbm") else load_robot(n) for n in names]
+    native.precompile(models)
     return names
 
 
@@ -74,7 +75,7 @@ def precompile_bundled():
     robot set, in parallel)."""
     import concurrent.futures as cf
     import multiprocessing as mp
-    sets = [(), ("fr3",), ("fetch",), ("fr3", "fetch")]
+    sets = [(), ("fr3",), ("fetch",), ("fr3", "fetch"), ("fr3_mbm",), ("fetch_mbm",), ("fr3_mbm", "fetch_mbm")]
     with cf.ProcessPoolExecutor(len(sets), mp_context=mp.get_context("spawn")) as ex:  # noqa: SIM117
         for names in ex.map(_precompile_one, sets):
             pass

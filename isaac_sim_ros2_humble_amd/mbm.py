@@ -97,3 +97,33 @@ def with_defaults(model, values):
         if n in values and not m.active[i]:
             m.defaults[i] = np.float32(values[n])
     return m
+
+
+# Inactive-joint values of every MBM request start state (uniform across the 14 problem sets).
+MBM_INACTIVE = {
+    "fetch": {"bellows_joint": 0.05, "head_pan_joint": 0.0, "head_tilt_joint": 0.0, "l_gripper_finger_joint": 0.05,
+              "l_wheel_joint": 0.0, "r_gripper_finger_joint": 0.05, "r_wheel_joint": 0.0},
+    "fr3": {"fr3_finger_joint1": 0.065, "fr3_finger_joint2": 0.065},
+}
+
+
+def mbm_model(model):
+    """The robot as configured by the MBM requests (inactive joints at the start-state values)."""
+    m = with_defaults(model, MBM_INACTIVE[model.name])
+    m.name = model.name + "_mbm"
+    return m
+
+
+def load_problems(root, models):
+    """All (set name, model index, Scene, start, goal) under `root`/<set>/scene*.yaml."""
+    import glob
+    import os
+    out = []
+    for d in sorted(os.listdir(root)):
+        if not os.path.isdir(os.path.join(root, d)):
+            continue
+        k = 1 if d.endswith("fetch") else 0
+        for sf in sorted(glob.glob(os.path.join(root, d, "scene*.yaml"))):
+            start, goal, _ = load_request(sf.replace("scene", "request"), models[k])
+            out.append((d, k, load_scene(sf), start, goal))
+    return out

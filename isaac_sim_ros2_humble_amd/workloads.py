@@ -213,7 +213,7 @@ def sample_pair(model, scene, rng, check, check_edge, want_free_line, batch=4096
     return None
 
 
-def c4_batch(models, n, check, check_edge, seed=0):
+def c4_batch(models, n, check, check_edge, seed=0, batch=4096):
     """n queries: (model_index, scene, start, goal, easy).  Queries whose scene admits no
     collision-free pair of the requested kind are redrawn with a new scene."""
     rng = np.random.default_rng(seed + 4)
@@ -223,7 +223,7 @@ def c4_batch(models, n, check, check_edge, seed=0):
         lo, hi, keep = workspace(models[k])
         scene = random_scene(rng, log_uniform_count(rng), lo, hi, keep)
         easy = (len(out) % 2) == 0
-        pair = sample_pair(models[k], scene, rng, check, check_edge, easy)
+        pair = sample_pair(models[k], scene, rng, check, check_edge, easy, batch=batch)
         if pair is None:
             continue
         out.append((k, scene, pair[0], pair[1], easy))

@@ -60,3 +60,11 @@ def test_mbm_start_goal_mostly_free_under_sphere_model(fetch, fr3):
             total += 2
     assert total == 140
     assert free / total > 0.5, free / total
+
+
+def test_mbm_inactive_constant_matches_requests(fetch, fr3):
+    for name in SETS:
+        model = fetch if name.endswith("fetch") else fr3
+        for rq in glob.glob(os.path.join(FIX, name, "request*.yaml")):
+            _, _, inactive = mbm.load_request(rq, model)
+            assert inactive == mbm.MBM_INACTIVE[model.name]
