@@ -208,7 +208,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--queries", type=int, default=256)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
-    ap.add_argument("--rate", type=float, nargs="*", default=[1000.0])
+    ap.add_argument("--rate", type=float, nargs="*", default=[100.0, 1000.0, 5000.0])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--c3-seeds", type=int, default=100)
     ap.add_argument("--skip", nargs="*", default=[], help="parts to skip: plain serve c2 c3 mbm cpu")
@@ -233,7 +233,7 @@ def main():
     # batched: warmup, then exactly K timed steps
     for _ in range(a.warmup):
         run_batch(rob_list, scenes, starts, goals, params)
-    step_ms, kern_ms, solved, flops, checked = [], [], [], [], []
+    step_ms, kern_ms, solved, flops, checked, host_ms = [], [], [], [], [], []
     for k in range(a.steps):
         params.seed = 1000 * (k + 1)
         res, ms, kms = run_batch(rob_list, scenes, starts, goals, params)
@@ -242,11 +242,21 @@ def main():
         solved.append(sum(r["status"] == "solved" for r in res))
         flops.append(sum(r["work_flops"] for r in res))
         checked.append(sum(r["configs_checked"] for r in res))
+        host_ms.append(res[0]["latency_ms"])  # the library's own submit-to-results time
         log("batch step %d: %.1f ms (kernel %.1f ms), solved %d/%d" % (k, ms, kms, solved[-1], len(res)))
     ms_per_step = float(np.mean(step_ms))
     batched_qps = float(np.sum(solved)) / (np.sum(step_ms) / 1e3)
     plan_kernel_ms = float(np.mean(kern_ms))
     achieved = float(np.mean(flops)) / (plan_kernel_ms / 1e3) / 1e12
+    iters = np.array([r["iterations"] for r in res])
+    # one more (untimed) step with the planner's phase profile on: where the waves spend their time
+    native.plan_profile(enable=True)
+    params.seed = 999
+    run_batch(rob_list, scenes, starts, goals, params)
+    prof = native.plan_profile(enable=False)
+    wave = max(prof["wave"], 1)
+    phase = {k: prof[k] / wave for k in native.PROFILE_PHASES[:8]}
+    phase.update({k: prof[k] for k in native.PROFILE_PHASES[9:]})
 
     out = {"metric": METRIC, "value": batched_qps, "unit": "queries/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
@@ -256,7 +266,10 @@ def main():
                                   "capacity 65536, resolution 0.02, one persistent launch per step" % len(queries),
                       "queries_per_step": len(queries), "tree_capacity": 65536, "resolution": 0.02},
            "batched": {"solved_per_step": solved, "success_rate": float(np.mean(solved)) / len(queries),
-                       "plan_kernel_ms": plan_kernel_ms, "configs_checked_per_step": float(np.mean(checked))},
+                       "plan_kernel_ms": plan_kernel_ms, "configs_checked_per_step": float(np.mean(checked)),
+                       "host_ms": float(np.mean(host_ms)), "iterations_p50_p99_max": [
+                           float(np.percentile(iters, 50)), float(np.percentile(iters, 99)), int(iters.max())],
+                       "phase_fraction_of_wave_time": phase},
            "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                         "kernel": "mp_plan (batched)", "flops_per_launch": float(np.mean(flops))}}

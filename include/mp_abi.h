@@ -59,6 +59,15 @@ double mp_last_kernel_ms(const char* family);
  * at the per-operation costs of the planning cost model) accumulated by the check / cost /
  * edge kernels since the last reset. */
 int mp_work_flops(uint64_t* out, int reset);
+/* Planner phase profile (diagnostic; no counterpart in the reference).  While enabled, planning
+ * launches add the wall-clock ticks (100 MHz) each wave spends per phase, and event counts, to
+ * library counters: [0] scheduling, [1] sample + nearest neighbour, [2] extension check,
+ * [3] node publication, [4] connection nearest neighbour, [5] connection checks, [6] path
+ * extraction, [7] whole iterations, [8] wave lifetime, [9] iterations, [10] inserts,
+ * [11] publication-wait sleeps.  enable: 1 on, 0 off, -1 unchanged.  counters (may be NULL)
+ * receives the first n (<= MP_PLAN_PROFILE_N) counters, which are then reset. */
+#define MP_PLAN_PROFILE_N 12
+int mp_plan_profile(int enable, uint64_t* counters, int n);
 
 /* ---------------------------------------------------------------------------------------- */
 /* Robot                                                                                    */

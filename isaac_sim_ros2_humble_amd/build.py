@@ -75,10 +75,17 @@ def precompile_bundled():
     robot set, in parallel)."""
     import concurrent.futures as cf
     import multiprocessing as mp
+    import glob
+    import time
     sets = [(), ("fr3",), ("fetch",), ("fr3", "fetch"), ("fr3_mbm",), ("fetch_mbm",), ("fr3_mbm", "fetch_mbm")]
+    t0 = time.time() - 1.0
     with cf.ProcessPoolExecutor(len(sets), mp_context=mp.get_context("spawn")) as ex:  # noqa: SIM117
         for names in ex.map(_precompile_one, sets):
             pass
+    # drop code objects of earlier sources (every bundled set was written or touched above)
+    for f in glob.glob(os.path.join(HERE, "_kcache", "*.hsaco")):
+        if os.path.getmtime(f) < t0:
+            os.remove(f)
 
 
 def build(precompile=True, verbose=False):

@@ -662,6 +662,10 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "    const int n_active = mp_uniform(ctl.ctl[2]);\n"
          "    const int budget = mp_uniform(ctl.ctl[3]);\n"
          "    if (n_active <= 0) return;\n"
+         "    __shared__ u64 prof_lds[MP_PROF_N];\n"
+         "    u64* pl = ctl.prof ? prof_lds : nullptr;\n"
+         "    if (pl && lane == 0) for (int k = 0; k < MP_PROF_N; ++k) pl[k] = 0ull;\n"
+         "    const u64 t_wave = mp_now(pl);\n"
          "    if (ctl.ctl[6]) {\n"
          "        // exclusive: wave w alone grows query active[w] (deterministic per seed)\n"
          "        const int w = (int)blockIdx.x;\n"
@@ -674,9 +678,13 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "            if (__hip_atomic_load(&st->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != MP_PLAN_RUNNING) break;\n"
          "            switch (Q.robot) {\n";
     for (size_t k = 0; k < robots.size(); k++)
-        o << "                case " << k << ": mp_plan_iteration<Robot" << k << ">(ctl, Q, st); break;\n";
-    o << "                default: break;\n            }\n        }\n        return;\n    }\n"
+        o << "                case " << k << ": mp_plan_iteration<Robot" << k << ">(ctl, Q, st, pl); break;\n";
+    o << "                default: break;\n            }\n        }\n"
+         "        mp_prof_add(pl, MP_PROF_WAVE, mp_now(pl) - t_wave);\n"
+         "        mp_prof_flush(ctl, pl);\n"
+         "        return;\n    }\n"
          "    for (;;) {\n"
+         "        u64 ts = mp_now(pl);\n"
          "        u32 t = 0;\n"
          "        if (lane == 0) t = __hip_atomic_fetch_add((u32*)ctl.ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
          "        t = (u32)__shfl((int)t, 0);\n"
@@ -699,14 +707,18 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "            if (b) { qi = __shfl(a, __builtin_ctzll(b)); break; }\n"
          "        }\n"
          "        qi = mp_uniform(qi);\n"
-         "        if (qi < 0) continue;\n"
+         "        if (qi < 0) { mp_prof_since(pl, MP_PROF_SCHED, ts); continue; }\n"
          "        QueryState* st = ctl.st + qi;\n"
          "        QueryDev Q;\n"
          "        mp_load_query(ctl.q + qi, Q);\n"
+         "        mp_prof_since(pl, MP_PROF_SCHED, ts);\n"
          "        switch (Q.robot) {\n";
     for (size_t k = 0; k < robots.size(); k++)
-        o << "            case " << k << ": mp_plan_iteration<Robot" << k << ">(ctl, Q, st); break;\n";
-    o << "            default: break;\n        }\n    }\n}\n";
+        o << "            case " << k << ": mp_plan_iteration<Robot" << k << ">(ctl, Q, st, pl); break;\n";
+    o << "            default: break;\n        }\n    }\n"
+         "    mp_prof_add(pl, MP_PROF_WAVE, mp_now(pl) - t_wave);\n"
+         "    mp_prof_flush(ctl, pl);\n"
+         "}\n";
     o << "extern \"C\" __global__ void __launch_bounds__(64) mp_plan_init(PlanCtl ctl) {\n"
          "    const int w = (int)blockIdx.x;\n"
          "    const int first = ctl.ctl[4], count = ctl.ctl[5];\n"

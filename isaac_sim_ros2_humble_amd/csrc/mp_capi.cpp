@@ -22,6 +22,8 @@ struct Global {
     hipStream_t stream = nullptr;
     int num_cus = 0;
     unsigned long long* work = nullptr;  // device counter of algorithmic flops (check/cost/edges)
+    unsigned long long* prof = nullptr;  // planner phase profile counters [MP_PLAN_PROFILE_N]
+    bool prof_on = false;
     std::mutex mu;
     std::map<std::string, std::pair<hipEvent_t, hipEvent_t>> events;
 } G;
@@ -53,6 +55,7 @@ namespace mp {
 hipStream_t lib_stream() { return G.stream; }
 int guard_call(const std::function<void()>& f) { return guard(f); }
 int lib_num_cus() { return G.num_cus; }
+unsigned long long* lib_plan_profile() { return G.prof_on ? G.prof : nullptr; }
 
 void launch(hipFunction_t f, unsigned grid, unsigned block, hipStream_t s, void** args, const char* family) {
     if (grid == 0) return;
@@ -138,6 +141,7 @@ RobotModel model_from_desc(const mp_robot_desc* d) {
                 m.psr.push_back(std::sqrt(thr));
             }
         }
+    m.hash = robot_hash(m);
     return m;
 }
 
@@ -161,6 +165,8 @@ int mp_init(int device) {
         G.num_cus = p.multiProcessorCount;
         MP_HIP(hipMalloc(&G.work, sizeof(unsigned long long)));
         MP_HIP(hipMemset(G.work, 0, sizeof(unsigned long long)));
+        MP_HIP(hipMalloc(&G.prof, MP_PLAN_PROFILE_N * sizeof(unsigned long long)));
+        MP_HIP(hipMemset(G.prof, 0, MP_PLAN_PROFILE_N * sizeof(unsigned long long)));
         G.device = device;
     });
 }
@@ -177,6 +183,9 @@ int mp_shutdown(void) {
         mp::plan_release_workspaces();
         if (G.work) hipFree(G.work);
         G.work = nullptr;
+        if (G.prof) hipFree(G.prof);
+        G.prof = nullptr;
+        G.prof_on = false;
         hipStreamDestroy(G.stream);
         G.stream = nullptr;
         G.device = -1;
@@ -230,6 +239,21 @@ int mp_stream_sync(void* stream) {
 }
 void* mp_default_stream(void) { return (void*)G.stream; }
 
+int mp_plan_profile(int enable, uint64_t* counters, int n) {
+    return guard([&] {
+        need_init();
+        if (counters && (n < 0 || n > MP_PLAN_PROFILE_N)) throw Error(MP_E_INVALID, "bad profile counter count");
+        MP_HIP(hipStreamSynchronize(G.stream));
+        if (counters) {
+            unsigned long long v[MP_PLAN_PROFILE_N];
+            MP_HIP(hipMemcpy(v, G.prof, sizeof(v), hipMemcpyDeviceToHost));
+            for (int k = 0; k < n; k++) counters[k] = v[k];
+            MP_HIP(hipMemset(G.prof, 0, sizeof(v)));
+        }
+        if (enable >= 0) G.prof_on = enable != 0;
+    });
+}
+
 int mp_work_flops(uint64_t* out, int reset) {
     return guard([&] {
         need_init();
@@ -262,7 +286,7 @@ int mp_robot_create(const mp_robot_desc* desc, mp_robot** out) {
         auto* r = new mp_robot();
         try {
             r->model = mp::model_from_desc(desc);
-            r->key = mp::robot_struct_source(r->model, "Robot0");
+            r->key = r->model.hash;
             r->module = mp::get_module({&r->model});
         } catch (...) {
             delete r;
@@ -279,12 +303,14 @@ int mp_precompile(const mp_robot_desc* const* descs, int n) {
         ms.reserve(n);
         for (int i = 0; i < n; i++) ms.push_back(mp::model_from_desc(descs[i]));
         std::vector<const mp::RobotModel*> ptrs;
+        for (auto& m : ms) ptrs.push_back(&m);
+        std::vector<const mp::RobotModel*> canon;
         std::string tag = "mp";
-        for (auto& m : ms) {
-            ptrs.push_back(&m);
-            tag += "_" + m.name;
+        for (int i : mp::canonical_order(ptrs)) {
+            canon.push_back(ptrs[i]);
+            tag += "_" + ptrs[i]->name;
         }
-        mp::rtc_compile(mp::module_source(ptrs), tag);
+        mp::rtc_compile(mp::module_source(canon), tag);
     });
 }
 

@@ -43,6 +43,7 @@ struct RobotModel {
     std::vector<float> act_lo, act_hi;
     std::vector<int> pi, pj;       // self pairs
     std::vector<float> pthr, psr;  // fl((ri+rj)^2), sqrtf(thr)
+    uint64_t hash = 0;             // robot_hash(*this), set once the model is complete
 };
 
 RobotModel model_from_desc(const mp_robot_desc* d);
@@ -70,7 +71,16 @@ struct Module {
     std::vector<std::pair<std::string, hipFunction_t>> fns_;
 };
 
-std::shared_ptr<Module> get_module(const std::vector<const RobotModel*>& robots);
+// A robot set maps to one code object whatever order callers list it in: the module's robot
+// slots follow the canonical order (robot name, then the hash of its generated struct).
+// canonical_order(...)[i] is the input index placed in module slot i.
+std::vector<int> canonical_order(const std::vector<const RobotModel*>& robots);
+// Loads (compiling on a cache miss) the module of a robot set; slot_of[i] receives the module
+// slot of robots[i].
+std::shared_ptr<Module> get_module(const std::vector<const RobotModel*>& robots, std::vector<int>* slot_of = nullptr);
+
+uint64_t fnv1a(const std::string& s, uint64_t h = 1469598103934665603ull);
+uint64_t robot_hash(const RobotModel& m);  // fnv1a of the robot's generated struct (computed)
 
 float sqrt_threshold(float x);  // smallest float y >= 0 with sqrtf(y) >= x
 
@@ -78,7 +88,7 @@ float sqrt_threshold(float x);  // smallest float y >= 0 with sqrtf(y) >= x
 
 struct mp_robot {
     mp::RobotModel model;
-    std::string key;  // hash of the generated robot struct
+    uint64_t key = 0;  // fnv1a of the generated robot struct
     std::shared_ptr<mp::Module> module;
 };
 

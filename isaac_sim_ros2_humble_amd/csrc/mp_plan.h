@@ -55,21 +55,26 @@ struct PlanCtlHost {
     int* ctl;
     float res;
     int max_step;
+    unsigned long long* prof;  // planner phase profile counters, null when off
 };
 
 hipStream_t lib_stream();
 int lib_num_cus();
+unsigned long long* lib_plan_profile();  // device counters when the phase profile is on, else null
 void launch(hipFunction_t f, unsigned grid, unsigned block, hipStream_t s, void** args, const char* family);
 SceneDevHost scene_view(const mp_scene* sc);
 void plan_release_workspaces();
 
-// Device buffers for `nslots` query slots: two SoA trees, two parent arrays and a path buffer
-// per slot, the query descriptors / states, the active list and the control block.
+// Buffers for `nslots` query slots: two SoA trees and two parent arrays per slot in device
+// memory, a path buffer per slot in pinned host memory that the kernel writes directly (a
+// finished path needs no copy of its own), the query descriptors / states, the active list and
+// the control block.
 struct SlotPool {
     int nslots = 0, cap = 0, dof_max = 0, max_path = 0;
     float* trees = nullptr;   // [nslots][2][dof_max][cap]
     int* parents = nullptr;   // [nslots][2][cap]
-    float* paths = nullptr;   // [nslots][max_path][dof_max]
+    float* hpaths = nullptr;  // pinned host [nslots][max_path * dof_max]; rows are the robot's dof wide
+    float* paths = nullptr;   // device address of hpaths
     QueryDevHost* dq = nullptr;
     QueryStateHost* dst = nullptr;
     int* dactive = nullptr;
@@ -86,12 +91,13 @@ struct SlotPool {
     PlanCtlHost ctl(float res, int max_step) const;
 };
 
-size_t slot_bytes(int cap, int dof_max, int max_path);
+size_t slot_bytes(int cap, int dof_max);  // device bytes of one slot
 
 // Runs f, converting exceptions to an ABI status code (message in mp_last_error()).
 int guard_call(const std::function<void()>& f);
 void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params& p, int waves, hipStream_t s);
-void harvest(SlotPool& pool, int slot, int dof, const QueryStateHost& st, mp_path* out, hipStream_t s);
+// Fills `out` from a finished query's state; call after the stream that ran the query has synchronised.
+void harvest(const SlotPool& pool, int slot, int dof, const QueryStateHost& st, mp_path* out);
 int default_waves(hipFunction_t plan_fn);
 
 }  // namespace mp

@@ -13,16 +13,19 @@ using mp::Error;
 
 namespace mp {
 
-size_t slot_bytes(int cap, int dof_max, int max_path) {
-    return (size_t)2 * dof_max * cap * sizeof(float) + (size_t)2 * cap * sizeof(int) +
-           (size_t)max_path * dof_max * sizeof(float);
+size_t slot_bytes(int cap, int dof_max) {
+    return (size_t)2 * dof_max * cap * sizeof(float) + (size_t)2 * cap * sizeof(int);
 }
 
 SlotPool::SlotPool(int ns, int c, int d, int mp_) : nslots(ns), cap(c), dof_max(d), max_path(mp_) {
     try {
         MP_HIP(hipMalloc(&trees, (size_t)nslots * 2 * dof_max * cap * sizeof(float)));
         MP_HIP(hipMalloc(&parents, (size_t)nslots * 2 * cap * sizeof(int)));
-        MP_HIP(hipMalloc(&paths, (size_t)nslots * max_path * dof_max * sizeof(float)));
+        // fine-grained (coherent) pinned memory: the planning kernel's path stores bypass the GPU
+        // caches and are in host memory once the kernel has finished
+        MP_HIP(hipHostMalloc(&hpaths, (size_t)nslots * max_path * dof_max * sizeof(float),
+                             hipHostMallocMapped | hipHostMallocCoherent));
+        MP_HIP(hipHostGetDevicePointer((void**)&paths, hpaths, 0));
         MP_HIP(hipMalloc(&dq, (size_t)nslots * sizeof(QueryDevHost)));
         MP_HIP(hipMalloc(&dst, (size_t)nslots * sizeof(QueryStateHost)));
         MP_HIP(hipMalloc(&dactive, (size_t)nslots * sizeof(int)));
@@ -41,7 +44,7 @@ SlotPool::SlotPool(int ns, int c, int d, int mp_) : nslots(ns), cap(c), dof_max(
 SlotPool::~SlotPool() {
     if (trees) hipFree(trees);
     if (parents) hipFree(parents);
-    if (paths) hipFree(paths);
+    if (hpaths) hipHostFree(hpaths);
     if (dq) hipFree(dq);
     if (dst) hipFree(dst);
     if (dactive) hipFree(dactive);
@@ -50,6 +53,7 @@ SlotPool::~SlotPool() {
     if (hst_pinned) hipHostFree(hst_pinned);
     trees = nullptr;
     parents = nullptr;
+    hpaths = nullptr;
     paths = nullptr;
     dq = nullptr;
     dst = nullptr;
@@ -59,7 +63,7 @@ SlotPool::~SlotPool() {
     hst_pinned = nullptr;
 }
 
-size_t SlotPool::bytes_per_slot() const { return slot_bytes(cap, dof_max, max_path); }
+size_t SlotPool::bytes_per_slot() const { return slot_bytes(cap, dof_max); }
 
 void SlotPool::fill_query(int slot, int robot_slot, int dof, const mp_scene* scene, const float* start,
                           const float* goal, uint64_t seed, int max_iters) {
@@ -91,6 +95,7 @@ PlanCtlHost SlotPool::ctl(float res, int max_step) const {
     c.ctl = dctl;
     c.res = res;
     c.max_step = max_step;
+    c.prof = lib_plan_profile();
     return c;
 }
 
@@ -165,7 +170,7 @@ void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params
     MP_HIP(hipStreamSynchronize(s));
 }
 
-void harvest(SlotPool& pool, int slot, int dof, const QueryStateHost& st, mp_path* out, hipStream_t s) {
+void harvest(const SlotPool& pool, int slot, int dof, const QueryStateHost& st, mp_path* out) {
     out->status = st.status < 0 ? MP_PLAN_RUNNING : st.status;
     out->iterations = std::min(st.sample_ctr, pool.hq[slot].max_iters);
     out->start_tree_size = std::min(st.pub[0], pool.cap);
@@ -180,16 +185,9 @@ void harvest(SlotPool& pool, int slot, int dof, const QueryStateHost& st, mp_pat
             return;
         }
         int n = std::min(st.path_len, out->capacity);
-        if (n > 0 && out->waypoints) {
-            // device path rows are dof_max wide; copy row-compacted
-            if (dof == pool.dof_max) {
-                MP_HIP(hipMemcpyAsync(out->waypoints, pool.hq[slot].path, (size_t)n * dof * sizeof(float),
-                                      hipMemcpyDeviceToHost, s));
-            } else {
-                MP_HIP(hipMemcpy2DAsync(out->waypoints, dof * sizeof(float), pool.hq[slot].path, dof * sizeof(float),
-                                        dof * sizeof(float), n, hipMemcpyDeviceToHost, s));
-            }
-        }
+        if (n > 0 && out->waypoints)
+            std::memcpy(out->waypoints, pool.hpaths + (size_t)slot * pool.max_path * pool.dof_max,
+                        (size_t)n * dof * sizeof(float));
         out->num_waypoints = n;
     }
 }
@@ -204,7 +202,7 @@ void solve_impl(const char* tag, int nq, mp_robot* const* robots, const mp_scene
     check_params(params);
     if (nq < 1 || !robots || !scenes || !starts16 || !goals16 || !out) throw Error(MP_E_INVALID, "bad batch arguments");
     double t0 = now_ms();
-    // robot set -> module (robots in order of first appearance)
+    // robot set -> module (distinct robots, mapped to the module's canonical slots)
     std::vector<mp_robot*> uniq;
     std::vector<int> slot_of(nq);
     int dof_max = 1;
@@ -226,7 +224,9 @@ void solve_impl(const char* tag, int nq, mp_robot* const* robots, const mp_scene
     } else {
         std::vector<const RobotModel*> ms;
         for (auto* r : uniq) ms.push_back(&r->model);
-        mod = get_module(ms);
+        std::vector<int> module_slot;
+        mod = get_module(ms, &module_slot);
+        for (int k = 0; k < nq; k++) slot_of[k] = module_slot[slot_of[k]];
     }
     hipStream_t s = lib_stream();
     SlotPool& pool = pool_for(tag, nq, params->tree_capacity, dof_max, params->max_path);
@@ -236,8 +236,7 @@ void solve_impl(const char* tag, int nq, mp_robot* const* robots, const mp_scene
     int waves = params->waves > 0 ? params->waves : (default_wave_count > 0 ? default_wave_count : default_waves(mod->fn("mp_plan")));
     if (params->exclusive) waves = nq;
     run_to_completion(*mod, pool, nq, *params, waves, s);
-    for (int k = 0; k < nq; k++) harvest(pool, k, robots[k]->model.dof, pool.hst_pinned[k], &out[k], s);
-    MP_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < nq; k++) harvest(pool, k, robots[k]->model.dof, pool.hst_pinned[k], &out[k]);
     double lat = now_ms() - t0;
     for (int k = 0; k < nq; k++) out[k].latency_ms = lat;
 }

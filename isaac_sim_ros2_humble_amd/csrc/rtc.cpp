@@ -7,10 +7,12 @@
 #include <hip/hiprtc.h>
 #include <sys/stat.h>
 #include <unistd.h>
+#include <utime.h>
 
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <algorithm>
 #include <map>
 #include <sstream>
 
@@ -34,14 +36,6 @@ const char* kOptions[] = {"--offload-arch=gfx950",
                           "-packed-fp32-ops"};
 const int kNumOptions = sizeof(kOptions) / sizeof(kOptions[0]);
 
-uint64_t fnv1a(const std::string& s, uint64_t h = 1469598103934665603ull) {
-    for (unsigned char c : s) {
-        h ^= c;
-        h *= 1099511628211ull;
-    }
-    return h;
-}
-
 std::string section(const char* begin, const char* end) {
     std::string t(kKernelTemplate);
     size_t a = t.find(begin), b = t.find(end);
@@ -52,7 +46,7 @@ std::string section(const char* begin, const char* end) {
 
 std::string lib_dir() {
     Dl_info info;
-    if (dladdr((void*)&fnv1a, &info) && info.dli_fname) {
+    if (dladdr((void*)&lib_dir, &info) && info.dli_fname) {
         std::string p(info.dli_fname);
         size_t k = p.rfind('/');
         if (k != std::string::npos) return p.substr(0, k);
@@ -60,6 +54,16 @@ std::string lib_dir() {
     return ".";
 }
 }  // namespace
+
+uint64_t fnv1a(const std::string& s, uint64_t h) {
+    for (unsigned char c : s) {
+        h ^= c;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+uint64_t robot_hash(const RobotModel& m) { return fnv1a(robot_struct_source(m, "Robot0")); }
 
 std::string kernel_template_prelude() { return section("//@@PRELUDE_BEGIN@@", "//@@PRELUDE_END@@"); }
 std::string kernel_template_body() { return section("//@@TEMPLATES_BEGIN@@", "//@@TEMPLATES_END@@"); }
@@ -85,7 +89,10 @@ std::vector<char> rtc_compile(const std::string& source, const std::string& tag)
         std::ifstream f(path, std::ios::binary);
         if (f) {
             std::vector<char> code((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
-            if (!code.empty()) return code;
+            if (!code.empty()) {
+                utime(path.c_str(), nullptr);  // mark as in use (the build prunes entries it did not touch)
+                return code;
+            }
         }
     }
     hiprtcProgram prog;
@@ -131,22 +138,55 @@ hipFunction_t Module::fn(const std::string& name) {
     return f;
 }
 
-std::shared_ptr<Module> get_module(const std::vector<const RobotModel*>& robots) {
+std::vector<int> canonical_order(const std::vector<const RobotModel*>& robots) {
+    std::vector<std::pair<std::string, uint64_t>> key(robots.size());
+    for (size_t i = 0; i < robots.size(); i++)
+        key[i] = {robots[i]->name, robots[i]->hash ? robots[i]->hash : robot_hash(*robots[i])};
+    std::vector<int> order(robots.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+    return order;
+}
+
+std::shared_ptr<Module> get_module(const std::vector<const RobotModel*>& robots, std::vector<int>* slot_of) {
     static std::mutex mu;
-    static std::map<std::string, std::weak_ptr<Module>> cache;
-    std::string src = module_source(robots);
-    std::string k = std::to_string(fnv1a(src));
+    // canonical robot hashes -> module.  Modules stay loaded (a multi-robot module is otherwise
+    // released after every batched call and reloaded by the next); the map is never destroyed,
+    // so no module is unloaded after the HIP runtime has shut down at process exit.
+    static auto* cache = new std::map<std::string, std::shared_ptr<Module>>();
+    static auto* order_lru = new std::vector<std::string>();
+    constexpr size_t kMaxModules = 16;
+    std::vector<int> order = canonical_order(robots);
+    if (slot_of) {
+        slot_of->assign(robots.size(), 0);
+        for (size_t s = 0; s < order.size(); s++) (*slot_of)[order[s]] = (int)s;
+    }
+    std::vector<const RobotModel*> canon;
+    std::string k;
+    for (int i : order) {
+        canon.push_back(robots[i]);
+        k += robots[i]->name + "#" + std::to_string(robots[i]->hash ? robots[i]->hash : robot_hash(*robots[i])) + ":";
+    }
     std::lock_guard<std::mutex> g(mu);
-    auto it = cache.find(k);
-    if (it != cache.end())
-        if (auto sp = it->second.lock()) return sp;
+    auto it = cache->find(k);
+    if (it != cache->end()) {
+        order_lru->erase(std::find(order_lru->begin(), order_lru->end(), k));
+        order_lru->push_back(k);
+        return it->second;
+    }
+    std::string src = module_source(canon);
     std::string tag = "mp";
-    for (auto* r : robots) tag += "_" + r->name;
+    for (auto* r : canon) tag += "_" + r->name;
     std::vector<char> code = rtc_compile(src, tag);
     auto m = std::make_shared<Module>();
     m->source = src;
     MP_HIP(hipModuleLoadData(&m->mod, code.data()));
-    cache[k] = m;
+    if (cache->size() >= kMaxModules) {  // drop the least recently used (users keep their shared_ptr)
+        cache->erase(order_lru->front());
+        order_lru->erase(order_lru->begin());
+    }
+    (*cache)[k] = m;
+    order_lru->push_back(k);
     return m;
 }
 

@@ -53,6 +53,7 @@ struct ActiveQ {
 struct mp_server {
     mp_server_params prm;
     std::vector<mp_robot*> robots;
+    std::vector<int> robot_slot;  // robot index -> the module's robot slot
     std::shared_ptr<mp::Module> mod;
     std::unique_ptr<mp::SlotPool> pool;
     int nslots = 0, waves = 0, budget = 0;
@@ -139,7 +140,7 @@ void mp_server::run() {
                     int slot = free_slots.back();
                     free_slots.pop_back();
                     mp_robot* r = robots[p.robot];
-                    pool->fill_query(slot, p.robot, r->model.dof, p.scene, p.start, p.goal, p.seed, prm.params.max_iterations);
+                    pool->fill_query(slot, robot_slot[p.robot], r->model.dof, p.scene, p.start, p.goal, p.seed, prm.params.max_iterations);
                     active[slot] = ActiveQ{p.ticket, p.t_submit, r->model.dof};
                     fresh.push_back(slot);
                 }
@@ -167,11 +168,10 @@ void mp_server::run() {
                 d.waypoints.resize((size_t)pool->max_path * aq.dof);
                 d.info.waypoints = d.waypoints.data();
                 d.info.capacity = pool->max_path;
-                mp::harvest(*pool, slot, aq.dof, st, &d.info, stream);
+                mp::harvest(*pool, slot, aq.dof, st, &d.info);
                 fin.emplace_back(aq.ticket, std::move(d));
                 freed.push_back(slot);
             }
-            if (!fin.empty()) MP_HIP(hipStreamSynchronize(stream));
             double t = now_ms();
             {
                 std::lock_guard<std::mutex> g(mu);
@@ -220,8 +220,13 @@ int mp_server_create(const mp_server_params* params, mp_server** out) {
             ms.push_back(&r->model);
             dof_max = std::max(dof_max, r->model.dof);
         }
-        s->mod = ms.size() == 1 ? s->robots[0]->module : mp::get_module(ms);
-        size_t per = mp::slot_bytes(params->tree_capacity, dof_max, p.max_path);
+        if (ms.size() == 1) {
+            s->mod = s->robots[0]->module;
+            s->robot_slot.assign(1, 0);
+        } else {
+            s->mod = mp::get_module(ms, &s->robot_slot);
+        }
+        size_t per = mp::slot_bytes(params->tree_capacity, dof_max);
         size_t fit = params->tree_memory_bytes / per;
         s->nslots = (int)std::min<size_t>((size_t)params->max_active, fit);
         if (s->nslots < 1) throw Error(MP_E_CAPACITY, "tree memory too small for one query");

@@ -67,6 +67,7 @@ _SIGS = {
     "mp_default_stream": (P, []),
     "mp_last_kernel_ms": (C.c_double, [C.c_char_p]),
     "mp_work_flops": (C.c_int, [C.POINTER(C.c_uint64), C.c_int]),
+    "mp_plan_profile": (C.c_int, [C.c_int, P, C.c_int]),
     "mp_robot_create": (C.c_int, [C.POINTER(RobotDesc), C.POINTER(C.c_void_p)]),
     "mp_robot_destroy": (C.c_int, [P]),
     "mp_robot_info": (C.c_int, [P, P, P, P, P]),
@@ -248,6 +249,18 @@ def work_flops(reset=True):
     return v.value
 
 
+PROFILE_PHASES = ["sched", "nn_extend", "check_extend", "insert", "nn_connect", "check_connect", "path", "iteration",
+                  "wave", "n_iterations", "n_inserts", "n_sleeps"]
+
+
+def plan_profile(enable=None):
+    """Reads and resets the planner phase counters (dict, ticks of 10 ns); enable: True/False/None."""
+    v = (C.c_uint64 * len(PROFILE_PHASES))()
+    e = -1 if enable is None else (1 if enable else 0)
+    check(lib().mp_plan_profile(e, C.cast(v, C.c_void_p), len(PROFILE_PHASES)), "mp_plan_profile")
+    return dict(zip(PROFILE_PHASES, [int(x) for x in v]))
+
+
 def kernel_ms(family):
     return lib().mp_last_kernel_ms(family.encode())
 
@@ -345,7 +358,7 @@ def _path_result(pth, wp, dof):
 
 def rrtc_solve(robot, scene, start, goal, params=None, max_path=4096):
     params = params or default_params()
-    wp = np.zeros((params.max_path, robot.dof), np.float32)
+    wp = np.empty((params.max_path, robot.dof), np.float32)
     pth = Path(0, 0, _p(wp), params.max_path, 0, 0, 0, 0, 0.0, 0.0)
     s = np.ascontiguousarray(start, np.float32)
     g = np.ascontiguousarray(goal, np.float32)
@@ -358,13 +371,15 @@ def rrtc_solve_batch(robots, scenes, starts, goals, params=None):
     nq = len(robots)
     s16 = np.zeros((nq, 16), np.float32)
     g16 = np.zeros((nq, 16), np.float32)
+    # one uninitialised block for every query's waypoints (only written rows are touched)
+    block = np.empty((nq, params.max_path * 16), np.float32)
     wps = []
     paths = (Path * nq)()
     for k in range(nq):
         d = robots[k].dof
         s16[k, :d] = starts[k]
         g16[k, :d] = goals[k]
-        wp = np.zeros((params.max_path, d), np.float32)
+        wp = block[k, :params.max_path * d].reshape(params.max_path, d)
         wps.append(wp)
         paths[k] = Path(0, 0, _p(wp), params.max_path, 0, 0, 0, 0, 0.0, 0.0)
     rh = (C.c_void_p * nq)(*[r.h for r in robots])

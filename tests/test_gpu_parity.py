@@ -26,6 +26,13 @@ def c2():
     return workloads.c2_scene(seed=1)
 
 
+@pytest.fixture(scope="module")
+def scenes(c2, fetch):
+    # the c2 scene crowds Fetch's base column (every Fetch configuration collides there);
+    # Fetch gets 40 mixed obstacles in its own workspace (about 75 % of uniform configs valid)
+    return {"fr3": c2, "fetch": workloads.random_scene(np.random.default_rng(1), 40, *workloads.workspace(fetch))}
+
+
 @pytest.mark.parametrize("name", ["fr3", "fetch"])
 def test_fk_bit_exact(name, gpu_robots, request, rng):
     m = request.getfixturevalue(name)
@@ -38,12 +45,12 @@ def test_fk_bit_exact(name, gpu_robots, request, rng):
 
 
 @pytest.mark.parametrize("name", ["fr3", "fetch"])
-def test_check_states_verdicts_identical(name, gpu_robots, request, c2, rng):
+def test_check_states_verdicts_identical(name, gpu_robots, request, scenes, rng):
     m = request.getfixturevalue(name)
     orc = request.getfixturevalue("orc_" + name)
     q = workloads.uniform_configs(m, 16387, rng)
-    v, cen = native.check_states(gpu_robots[name], native.Scene(c2), q, want_centres=True)
-    vo, _, _, _ = oracle.check(orc, oracle.OracleScene(c2), q, nthreads=8)
+    v, cen = native.check_states(gpu_robots[name], native.Scene(scenes[name]), q, want_centres=True)
+    vo, _, _, _ = oracle.check(orc, oracle.OracleScene(scenes[name]), q, nthreads=8)
     mism = np.nonzero(v != vo)[0]
     assert len(mism) == 0, "verdict mismatches at %s" % mism[:10]
     assert np.array_equal(cen, orc.fk32(q))
@@ -51,16 +58,18 @@ def test_check_states_verdicts_identical(name, gpu_robots, request, c2, rng):
 
 
 @pytest.mark.parametrize("name", ["fr3", "fetch"])
-def test_costs_identical(name, gpu_robots, request, c2, rng):
+def test_costs_identical(name, gpu_robots, request, scenes, rng):
     m = request.getfixturevalue(name)
     orc = request.getfixturevalue("orc_" + name)
     q = workloads.uniform_configs(m, 2051, rng)
-    cost, v = native.state_costs(gpu_robots[name], native.Scene(c2), q)
-    vo, co, _, _ = oracle.check(orc, oracle.OracleScene(c2), q, want_cost=True, nthreads=8)
+    sc = scenes[name]
+    cost, v = native.state_costs(gpu_robots[name], native.Scene(sc), q)
+    vo, co, _, _ = oracle.check(orc, oracle.OracleScene(sc), q, want_cost=True, nthreads=8)
     assert np.array_equal(v, vo)
     assert np.array_equal(cost, co)
+    assert 0 < v.sum() < len(v)
     # the early-out verdict kernel agrees with the full evaluation
-    assert np.array_equal(native.check_states(gpu_robots[name], native.Scene(c2), q), v)
+    assert np.array_equal(native.check_states(gpu_robots[name], native.Scene(sc), q), v)
 
 
 def test_contact_band_cases():

