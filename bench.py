@@ -131,6 +131,19 @@ def cpu_baseline(models, queries, budget_s, threads):
                                                                       solved, dt)}
 
 
+def cpu_checker_c2(model, threads, n=1 << 16):
+    """The CPU FP32 FK + collision checker (oracle) on the first n of the 2^20 C2 configurations."""
+    import oracle
+    scene = workloads.c2_scene(seed=1)
+    q = workloads.c2_configs(model, 1 << 20, seed=1)[:n]
+    orc, osc = oracle.OracleRobot(model), oracle.OracleScene(scene)
+    t0 = time.perf_counter()
+    oracle.check(orc, osc, q, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "configs/s", "cores": threads, "kind": "port",
+            "sample": "first %d of the 2^20 C2 configurations (FR3, 64 obstacles), verdict mode" % n}
+
+
 def bench_c3(robot, model, seeds):
     """Fetch 8-DOF in a 3-shelf unit: start outside, goal between shelves, one query per seed."""
     scene, shelf = workloads.c3_scene(seed=0)
@@ -211,7 +224,8 @@ def main():
     ap.add_argument("--rate", type=float, nargs="*", default=[100.0, 1000.0, 5000.0])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--c3-seeds", type=int, default=100)
-    ap.add_argument("--skip", nargs="*", default=[], help="parts to skip: plain serve c2 c3 mbm cpu")
+    ap.add_argument("--tune-queries", type=int, default=128)
+    ap.add_argument("--skip", nargs="*", default=[], help="parts to skip: tune plain serve c2 c3 mbm cpu")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: queries shard across GPUs with no exchange, so the "
@@ -228,6 +242,22 @@ def main():
     scenes = [native.Scene(sc) for _, sc, _, _, _ in queries]
     starts = [s for _, _, s, _, _ in queries]
     goals = [g for _, _, _, g, _ in queries]
+
+    # autotune the launch shapes (team size x grid) on a separate calibration workload; the
+    # choice is stored per robot set and class and used by the batched, plain and served runs
+    tune = {}
+    if "tune" not in a.skip:
+        cal = build_c4(robots, models, a.tune_queries, seed=1)
+        c_rob = [robots[models[k].name] for k, _, _, _, _ in cal]
+        c_sc = [native.Scene(sc) for _, sc, _, _, _ in cal]
+        c_s = [q[2] for q in cal]
+        c_g = [q[3] for q in cal]
+        for name, per_query, n in (("batch", False, len(cal)), ("plain", True, min(32, len(cal)))):
+            best, cands = native.rrtc_autotune(c_rob[:n], c_sc[:n], c_s[:n], c_g[:n], native.default_params(seed=77),
+                                               per_query=per_query, reps=2)
+            tune[name] = {"team_waves": best.team_waves, "waves": best.waves, "calibration_queries": n,
+                          "candidates": cands}
+            log("autotune %s: team_waves %d waves %d" % (name, best.team_waves, best.waves))
     params = native.default_params(seed=0)
 
     # batched: warmup, then exactly K timed steps
@@ -249,6 +279,7 @@ def main():
     plan_kernel_ms = float(np.mean(kern_ms))
     achieved = float(np.mean(flops)) / (plan_kernel_ms / 1e3) / 1e12
     iters = np.array([r["iterations"] for r in res])
+    dev_ms = np.array([r["device_ms"] for r in res])
     # one more (untimed) step with the planner's phase profile on: where the waves spend their time
     native.plan_profile(enable=True)
     params.seed = 999
@@ -269,10 +300,13 @@ def main():
                        "plan_kernel_ms": plan_kernel_ms, "configs_checked_per_step": float(np.mean(checked)),
                        "host_ms": float(np.mean(host_ms)), "iterations_p50_p99_max": [
                            float(np.percentile(iters, 50)), float(np.percentile(iters, 99)), int(iters.max())],
+                       "query_device_ms_p50_p99_max": [float(np.percentile(dev_ms, 50)),
+                                                       float(np.percentile(dev_ms, 99)), float(dev_ms.max())],
                        "phase_fraction_of_wave_time": phase},
            "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                        "kernel": "mp_plan (batched)", "flops_per_launch": float(np.mean(flops))}}
+                        "kernel": "mp_plan (batched)", "flops_per_launch": float(np.mean(flops))},
+           "autotune": tune}
 
     if "plain" not in a.skip:
         t0 = time.perf_counter()
@@ -313,6 +347,8 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(models, queries, a.cpu_seconds, threads)
         log("cpu:", out["cpu_baseline"])
+        out["cpu_checker_c2"] = cpu_checker_c2(models[0], threads)
+        log("cpu c2:", out["cpu_checker_c2"])
     print(json.dumps(out), flush=True)
 
 

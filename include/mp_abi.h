@@ -64,7 +64,7 @@ int mp_work_flops(uint64_t* out, int reset);
  * library counters: [0] scheduling, [1] sample + nearest neighbour, [2] extension check,
  * [3] node publication, [4] connection nearest neighbour, [5] connection checks, [6] path
  * extraction, [7] whole iterations, [8] wave lifetime, [9] iterations, [10] inserts,
- * [11] publication-wait sleeps.  enable: 1 on, 0 off, -1 unchanged.  counters (may be NULL)
+ * [11] reserved (0).  enable: 1 on, 0 off, -1 unchanged.  counters (may be NULL)
  * receives the first n (<= MP_PLAN_PROFILE_N) counters, which are then reset. */
 #define MP_PLAN_PROFILE_N 12
 int mp_plan_profile(int enable, uint64_t* counters, int n);
@@ -184,12 +184,15 @@ typedef struct {
     int waves;             /* persistent grid size in waves (0 = tuned default) */
     int max_path;          /* waypoint capacity per query */
     uint64_t seed;         /* sample stream seed; query k uses seed + k */
-    int exclusive;         /* 1: one wave grows each query (deterministic per seed, identical
-                              alone and in a batch); 0: all waves share all queries */
+    int exclusive;         /* 1: one team grows each query (deterministic per seed, identical
+                              alone and in a batch); 0: all teams share all queries */
+    int team_waves;        /* waves per team (1, 2, 4 or 8; 0 = tuned default): the waves of a
+                              team share one iteration's nearest-neighbour scans and collision
+                              checks (threads per tree iteration = 64 * team_waves) */
 } mp_rrtc_params;
 
 /* Defaults: resolution 0.02, max_step 32, max_iterations 10000, tree_capacity 65536,
- * waves 0, max_path 4096, seed 0, exclusive 0. */
+ * waves 0, max_path 4096, seed 0, exclusive 0, team_waves 0. */
 void mp_rrtc_default_params(mp_rrtc_params* p);
 
 #define MP_PLAN_RUNNING 0
@@ -209,6 +212,7 @@ typedef struct {
     int64_t configs_checked;
     double latency_ms;     /* submit -> result on host */
     double work_flops;     /* algorithmic flops of the collision checks run for this query */
+    double device_ms;      /* device time from admission (start/goal check) to the final status */
 } mp_path;
 
 /* One query, one launch: the CPRRTC::RobotSolver::solve equivalent (RobotSolver.cu:348-473):
@@ -221,6 +225,24 @@ int mp_rrtc_solve(mp_robot* robot, const mp_scene* scene, const float* start, co
  * queues; robots[k], scenes[k], starts[k] / goals[k] ([dof_k] floats at offset k*16). */
 int mp_rrtc_solve_batch(int nq, mp_robot* const* robots, const mp_scene* const* scenes, const float* starts16,
                         const float* goals16, const mp_rrtc_params* params, mp_path* out);
+
+/* Autotuner of the planning launch shape (no counterpart in the reference; CPRRTC fixes one
+ * block of threads per worker).  Solves the sample queries once per candidate shape and seed
+ * (team_waves 2, 4, 8 x total waves; per_query = 1 tunes the one-query-per-launch shape by
+ * solving the sample one query at a time, 0 tunes the batch shape with the sample as one batch),
+ * scores each by its mean planning-kernel time, and returns the fastest in *best (params with
+ * team_waves and waves set).  The choice is kept for this robot set and class, in memory and in
+ * the code-object cache directory, and used by every later solve or server whose params leave
+ * team_waves and waves at 0.  results (may be NULL) receives up to cap candidates. */
+typedef struct {
+    int team_waves;
+    int waves;             /* 0 = every team the device keeps resident */
+    double kernel_ms;      /* mean planning-kernel time over the seeds (summed over queries
+                              when per_query) */
+} mp_tune_result;
+int mp_rrtc_autotune(int nq, mp_robot* const* robots, const mp_scene* const* scenes, const float* starts16,
+                     const float* goals16, const mp_rrtc_params* params, int per_query, int reps,
+                     mp_tune_result* results, int cap, int* n_results, mp_rrtc_params* best);
 
 /* ---------------------------------------------------------------------------------------- */
 /* Serving: continuous batching                                                             */

@@ -393,10 +393,14 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
     };
     auto self_body = [&](bool cost) {
         Emitter X;
+        int blk = 0;
         for (const LP& lp : lps) {
             const SphereLink& A = SL[lp.a];
             const SphereLink& B = SL[lp.b];
             float cull = (float)((A.R + B.R + kCullMargin) * (A.R + B.R + kCullMargin));
+            // in a team, wave w evaluates the blocks blk with blk % W == w
+            if (!cost) X.os << "        if (T.mine(" << blk << ")) {\n";
+            blk++;
             X.os << "        {\n";
             V d[3];
             for (int a = 0; a < 3; a++) d[a] = X.sub(FV[lp.a][12 + a], FV[lp.b][12 + a]);
@@ -422,7 +426,7 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
                     X.os << "            h |= " << X.ref(dd) << " < " << lit(std::min(thr, sqrt_threshold(sr))) << ";\n";
             }
             X.os << "        }\n        }\n";
-            if (!cost) X.os << "        if (mp_stop<MODE>(h, mask)) return h;\n";
+            if (!cost) X.os << "        if (mp_stop<MODE>(h, mask) || T.stopped()) return h;\n        }\n";
         }
         return X.os.str();
     };
@@ -548,7 +552,7 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
     o << "    static constexpr unsigned FK_WORK = " << fk_flops << "u;\n";
     o << "    static __device__ __forceinline__ void fk(const float* q, float* F) {\n" << fk_body << "    }\n";
     o << "    static __device__ __forceinline__ void write_centres(const float* F, float* out) {\n" << centres_body << "    }\n";
-    o << "    template <int MODE>\n    static __device__ __forceinline__ bool self_hit(const float* F, bool h, u64 mask, unsigned& work) {\n"
+    o << "    template <int MODE>\n    static __device__ __forceinline__ bool self_hit(const float* F, bool h, u64 mask, unsigned& work, const Team& T) {\n"
       << self_body(false) << "        return h;\n    }\n";
     o << "    static __device__ __forceinline__ float self_cost(const float* F, unsigned& work) {\n        float part = 0.0f;\n"
       << self_body(true) << "        return part;\n    }\n";
@@ -574,15 +578,17 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
         "            const float m8 = mp_sld(b + 8), m9 = mp_sld(b + 9), m10 = mp_sld(b + 10), m11 = mp_sld(b + 11);\n"
         "            const float crad = mp_sld(b + 12), hh = mp_sld(b + 13);\n";
     // early-out order: boxes, spheres, capsules, cylinders (cheapest per pair first)
-    o << "    template <int MODE>\n    static __device__ __forceinline__ bool env_hit(const float* F, const SceneDev& sc, bool hit, u64 mask, unsigned& work) {\n";
-    o << "        for (int o = 0; o < mp_uniform(sc.nb); ++o) {\n" << box_load << env_block(0, false)
-      << "            if (mp_stop<MODE>(hit, mask)) return hit;\n        }\n";
-    o << "        for (int o = 0; o < mp_uniform(sc.ns); ++o) {\n" << sph_load << env_block(1, false)
-      << "            if (mp_stop<MODE>(hit, mask)) return hit;\n        }\n";
-    o << "        for (int o = 0; o < mp_uniform(sc.ncap); ++o) {\n" << cap_load << env_block(2, false)
-      << "            if (mp_stop<MODE>(hit, mask)) return hit;\n        }\n";
-    o << "        for (int o = 0; o < mp_uniform(sc.nc); ++o) {\n" << cyl_load << env_block(3, false)
-      << "            if (mp_stop<MODE>(hit, mask)) return hit;\n        }\n";
+    // in a team, wave w takes the obstacles o with o % W == w of every kind
+    o << "    template <int MODE>\n    static __device__ __forceinline__ bool env_hit(const float* F, const SceneDev& sc, bool hit, u64 mask, unsigned& work, const Team& T) {\n";
+    o << "        const int o0 = mp_uniform(T.w), os = mp_uniform(T.W);\n";
+    o << "        for (int o = o0; o < mp_uniform(sc.nb); o += os) {\n" << box_load << env_block(0, false)
+      << "            if (mp_stop<MODE>(hit, mask) || T.stopped()) return hit;\n        }\n";
+    o << "        for (int o = o0; o < mp_uniform(sc.ns); o += os) {\n" << sph_load << env_block(1, false)
+      << "            if (mp_stop<MODE>(hit, mask) || T.stopped()) return hit;\n        }\n";
+    o << "        for (int o = o0; o < mp_uniform(sc.ncap); o += os) {\n" << cap_load << env_block(2, false)
+      << "            if (mp_stop<MODE>(hit, mask) || T.stopped()) return hit;\n        }\n";
+    o << "        for (int o = o0; o < mp_uniform(sc.nc); o += os) {\n" << cyl_load << env_block(3, false)
+      << "            if (mp_stop<MODE>(hit, mask) || T.stopped()) return hit;\n        }\n";
     o << "        return hit;\n    }\n";
     // full-evaluation costs in the oracle order: spheres, boxes, cylinders, capsules; culled
     // blocks contribute exact zeros, so skipping them leaves every partial sum unchanged
@@ -657,56 +663,77 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "    Q.seed = mp_sld(&p->seed); Q.robot = mp_sld(&p->robot); Q.dof = mp_sld(&p->dof); Q.cap = mp_sld(&p->cap);\n"
          "    Q.max_iters = mp_sld(&p->max_iters); Q.max_path = mp_sld(&p->max_path);\n"
          "}\n";
-    o << "extern \"C\" __global__ void __launch_bounds__(64) mp_plan(PlanCtl ctl) {\n"
+    // planning kernels: one team (workgroup of W waves, W = blockDim.x / 64) per query iteration
+    const std::string team_setup =
+        "    __shared__ u64 team_red[MP_TEAM_MAX];\n"
+        "    __shared__ int team_flag;\n"
+        "    Team T;\n"
+        "    T.w = mp_uniform((int)(threadIdx.x >> 6));\n"
+        "    T.W = mp_uniform((int)(blockDim.x >> 6));\n"
+        "    T.red = team_red;\n"
+        "    T.flag = &team_flag;\n"
+        "    if (threadIdx.x == 0) team_flag = 0;\n";
+    o << "extern \"C\" __global__ void __launch_bounds__(64 * MP_TEAM_MAX) mp_plan(PlanCtl ctl) {\n"
          "    const int lane = threadIdx.x & (MP_WAVE - 1);\n"
          "    const int n_active = mp_uniform(ctl.ctl[2]);\n"
          "    const int budget = mp_uniform(ctl.ctl[3]);\n"
          "    if (n_active <= 0) return;\n"
+      << team_setup <<
+         "    const bool lead = T.w == 0;\n"
          "    __shared__ u64 prof_lds[MP_PROF_N];\n"
-         "    u64* pl = ctl.prof ? prof_lds : nullptr;\n"
+         "    u64* pl = (ctl.prof && lead) ? prof_lds : nullptr;\n"
          "    if (pl && lane == 0) for (int k = 0; k < MP_PROF_N; ++k) pl[k] = 0ull;\n"
+         "    __syncthreads();\n"
          "    const u64 t_wave = mp_now(pl);\n"
          "    if (ctl.ctl[6]) {\n"
-         "        // exclusive: wave w alone grows query active[w] (deterministic per seed)\n"
-         "        const int w = (int)blockIdx.x;\n"
-         "        if (w >= n_active) return;\n"
-         "        const int qi = ctl.active[w];\n"
+         "        // exclusive: team b alone grows query active[b] (deterministic per seed)\n"
+         "        const int b = (int)blockIdx.x;\n"
+         "        if (b >= n_active) return;\n"
+         "        const int qi = ctl.active[b];\n"
          "        QueryState* st = ctl.st + qi;\n"
          "        QueryDev Q;\n"
          "        mp_load_query(ctl.q + qi, Q);\n"
          "        for (int it = 0; budget <= 0 || it < budget; ++it) {\n"
-         "            if (__hip_atomic_load(&st->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != MP_PLAN_RUNNING) break;\n"
+         "            const int status = mp_team_bcast(lead ? __hip_atomic_load(&st->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0, T);\n"
+         "            if (status != MP_PLAN_RUNNING) break;\n"
          "            switch (Q.robot) {\n";
     for (size_t k = 0; k < robots.size(); k++)
-        o << "                case " << k << ": mp_plan_iteration<Robot" << k << ">(ctl, Q, st, pl); break;\n";
+        o << "                case " << k << ": mp_plan_iteration<Robot" << k << ">(ctl, Q, st, pl, T); break;\n";
     o << "                default: break;\n            }\n        }\n"
          "        mp_prof_add(pl, MP_PROF_WAVE, mp_now(pl) - t_wave);\n"
          "        mp_prof_flush(ctl, pl);\n"
          "        return;\n    }\n"
          "    for (;;) {\n"
          "        u64 ts = mp_now(pl);\n"
-         "        u32 t = 0;\n"
-         "        if (lane == 0) t = __hip_atomic_fetch_add((u32*)ctl.ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
-         "        t = (u32)__shfl((int)t, 0);\n"
-         "        if (budget > 0 && t >= (u32)budget) break;\n"
-         "        if (__hip_atomic_load(ctl.ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= 0) break;\n"
-         "        // round-robin over the active list; skip finished queries with a wave-wide probe\n"
-         "        const int slot = (int)(t % (u32)n_active);\n"
-         "        int qi = -1;\n"
-         "        for (int base = 0; base < n_active; base += MP_WAVE) {\n"
-         "            const int p = base + lane;\n"
-         "            int a = -1;\n"
-         "            bool run = false;\n"
-         "            if (p < n_active) {\n"
-         "                int k = slot + p;\n"
-         "                if (k >= n_active) k -= n_active;\n"
-         "                a = ctl.active[k];\n"
-         "                run = __hip_atomic_load(&ctl.st[a].status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == MP_PLAN_RUNNING;\n"
+         "        // the lead wave takes a ticket and picks the query: round-robin over the active list,\n"
+         "        // skipping finished queries with a wave-wide probe (-2: the launch is done)\n"
+         "        int qi = -2;\n"
+         "        if (lead) {\n"
+         "            u32 t = 0;\n"
+         "            if (lane == 0) t = __hip_atomic_fetch_add((u32*)ctl.ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "            t = (u32)__shfl((int)t, 0);\n"
+         "            const int running = mp_uniform(__hip_atomic_load(ctl.ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));\n"
+         "            if (!(budget > 0 && t >= (u32)budget) && running > 0) {\n"
+         "                qi = -1;\n"
+         "                const int slot = (int)(t % (u32)n_active);\n"
+         "                for (int base = 0; base < n_active; base += MP_WAVE) {\n"
+         "                    const int p = base + lane;\n"
+         "                    int a = -1;\n"
+         "                    bool run = false;\n"
+         "                    if (p < n_active) {\n"
+         "                        int k = slot + p;\n"
+         "                        if (k >= n_active) k -= n_active;\n"
+         "                        a = ctl.active[k];\n"
+         "                        run = __hip_atomic_load(&ctl.st[a].status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == MP_PLAN_RUNNING;\n"
+         "                    }\n"
+         "                    const u64 b = __ballot(run);\n"
+         "                    if (b) { qi = __shfl(a, __builtin_ctzll(b)); break; }\n"
+         "                }\n"
+         "                qi = mp_uniform(qi);\n"
          "            }\n"
-         "            const u64 b = __ballot(run);\n"
-         "            if (b) { qi = __shfl(a, __builtin_ctzll(b)); break; }\n"
          "        }\n"
-         "        qi = mp_uniform(qi);\n"
+         "        qi = mp_team_bcast(qi, T);\n"
+         "        if (qi == -2) break;\n"
          "        if (qi < 0) { mp_prof_since(pl, MP_PROF_SCHED, ts); continue; }\n"
          "        QueryState* st = ctl.st + qi;\n"
          "        QueryDev Q;\n"
@@ -714,22 +741,24 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "        mp_prof_since(pl, MP_PROF_SCHED, ts);\n"
          "        switch (Q.robot) {\n";
     for (size_t k = 0; k < robots.size(); k++)
-        o << "            case " << k << ": mp_plan_iteration<Robot" << k << ">(ctl, Q, st, pl); break;\n";
+        o << "            case " << k << ": mp_plan_iteration<Robot" << k << ">(ctl, Q, st, pl, T); break;\n";
     o << "            default: break;\n        }\n    }\n"
          "    mp_prof_add(pl, MP_PROF_WAVE, mp_now(pl) - t_wave);\n"
          "    mp_prof_flush(ctl, pl);\n"
          "}\n";
-    o << "extern \"C\" __global__ void __launch_bounds__(64) mp_plan_init(PlanCtl ctl) {\n"
-         "    const int w = (int)blockIdx.x;\n"
+    o << "extern \"C\" __global__ void __launch_bounds__(64 * MP_TEAM_MAX) mp_plan_init(PlanCtl ctl) {\n"
+         "    const int b = (int)blockIdx.x;\n"
          "    const int first = ctl.ctl[4], count = ctl.ctl[5];\n"
-         "    if (w >= count) return;\n"
-         "    const int qi = ctl.active[first + w];\n"
+         "    if (b >= count) return;\n"
+      << team_setup <<
+         "    __syncthreads();\n"
+         "    const int qi = ctl.active[first + b];\n"
          "    QueryDev Q;\n"
          "    mp_load_query(ctl.q + qi, Q);\n"
          "    for (int j = 0; j < 16; ++j) { Q.start[j] = ctl.q[qi].start[j]; Q.goal[j] = ctl.q[qi].goal[j]; }\n"
          "    switch (Q.robot) {\n";
     for (size_t k = 0; k < robots.size(); k++)
-        o << "        case " << k << ": mp_plan_init_one<Robot" << k << ">(ctl, Q, ctl.st + qi); break;\n";
+        o << "        case " << k << ": mp_plan_init_one<Robot" << k << ">(ctl, Q, ctl.st + qi, T); break;\n";
     o << "        default: break;\n    }\n}\n";
     return o.str();
 }

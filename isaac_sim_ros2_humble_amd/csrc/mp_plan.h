@@ -27,7 +27,7 @@ struct QueryDevHost {
     float* path;
     uint64_t seed;
     int robot, dof, cap, max_iters;
-    int max_path, pad0, pad1, pad2;
+    int max_path, dof_max, pad1, pad2;
     float start[16];
     float goal[16];
 };
@@ -35,7 +35,6 @@ static_assert(sizeof(QueryDevHost) == 256, "QueryDev layout");
 
 struct QueryStateHost {
     int alloc[2];
-    int pub[2];
     int sample_ctr;
     int status;
     int conn[2];
@@ -43,8 +42,9 @@ struct QueryStateHost {
     int iters;
     uint64_t checked;
     uint64_t flops;
+    uint64_t t_init, t_done;
 };
-static_assert(sizeof(QueryStateHost) == 56, "QueryState layout");
+static_assert(sizeof(QueryStateHost) == 64, "QueryState layout");
 
 // control block in device memory: [0] ticket, [1] remaining, [2] n_active, [3] budget,
 // [4] first new query (index into the active list), [5] number of new queries
@@ -67,8 +67,9 @@ void plan_release_workspaces();
 
 // Buffers for `nslots` query slots: two SoA trees and two parent arrays per slot in device
 // memory, a path buffer per slot in pinned host memory that the kernel writes directly (a
-// finished path needs no copy of its own), the query descriptors / states, the active list and
-// the control block.
+// finished path needs no copy of its own; the kernel orders the path before the query's status
+// at system scope and the host reads it after the stream has synchronised), the query
+// descriptors / states, the active list and the control block.
 struct SlotPool {
     int nslots = 0, cap = 0, dof_max = 0, max_path = 0;
     float* trees = nullptr;   // [nslots][2][dof_max][cap]
@@ -81,7 +82,7 @@ struct SlotPool {
     int* dctl = nullptr;      // control block (8 ints)
     int* hpinned = nullptr;   // pinned staging: control block (8) + active list (nslots)
     QueryStateHost* hst_pinned = nullptr;
-    std::vector<QueryDevHost> hq;
+    QueryDevHost* hq = nullptr;  // pinned host staging of the descriptors [nslots]
 
     SlotPool(int nslots, int cap, int dof_max, int max_path);
     ~SlotPool();
@@ -95,9 +96,22 @@ size_t slot_bytes(int cap, int dof_max);  // device bytes of one slot
 
 // Runs f, converting exceptions to an ABI status code (message in mp_last_error()).
 int guard_call(const std::function<void()>& f);
-void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params& p, int waves, hipStream_t s);
+void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params& p, int team, int teams,
+                       hipStream_t s);
+
+// Launch shape of the planning kernel: waves per team and total waves (0: resident teams).
+struct Shape {
+    int team = 0;
+    int waves = 0;
+};
+std::string robot_set_key(const std::vector<mp_robot*>& robots);
+Shape tuned_shape(const std::string& set_key, int cls);  // team 0 when nothing is tuned
+void store_shape(const std::string& set_key, int cls, Shape sh);
 // Fills `out` from a finished query's state; call after the stream that ran the query has synchronised.
 void harvest(const SlotPool& pool, int slot, int dof, const QueryStateHost& st, mp_path* out);
-int default_waves(hipFunction_t plan_fn);
+constexpr int kDefaultTeamWaves = 4;
+// Resident teams of `team_waves` waves the planning kernel can keep on the device.
+int default_teams(hipFunction_t plan_fn, int team_waves);
+int team_size(const mp_rrtc_params& p);  // waves per team (the default when p.team_waves is 0)
 
 }  // namespace mp

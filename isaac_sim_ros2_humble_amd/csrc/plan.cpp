@@ -3,6 +3,7 @@
 // batched persistent solve (many queries, one launch).
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -21,10 +22,10 @@ SlotPool::SlotPool(int ns, int c, int d, int mp_) : nslots(ns), cap(c), dof_max(
     try {
         MP_HIP(hipMalloc(&trees, (size_t)nslots * 2 * dof_max * cap * sizeof(float)));
         MP_HIP(hipMalloc(&parents, (size_t)nslots * 2 * cap * sizeof(int)));
-        // fine-grained (coherent) pinned memory: the planning kernel's path stores bypass the GPU
-        // caches and are in host memory once the kernel has finished
+        // non-coherent pinned memory: host reads are cached (fine-grained host memory reads
+        // uncached, which made harvesting a large batch cost milliseconds)
         MP_HIP(hipHostMalloc(&hpaths, (size_t)nslots * max_path * dof_max * sizeof(float),
-                             hipHostMallocMapped | hipHostMallocCoherent));
+                             hipHostMallocMapped | hipHostMallocNonCoherent));
         MP_HIP(hipHostGetDevicePointer((void**)&paths, hpaths, 0));
         MP_HIP(hipMalloc(&dq, (size_t)nslots * sizeof(QueryDevHost)));
         MP_HIP(hipMalloc(&dst, (size_t)nslots * sizeof(QueryStateHost)));
@@ -32,13 +33,17 @@ SlotPool::SlotPool(int ns, int c, int d, int mp_) : nslots(ns), cap(c), dof_max(
         MP_HIP(hipMalloc(&dctl, 16 * sizeof(int)));
         MP_HIP(hipHostMalloc(&hpinned, (size_t)(16 + nslots) * sizeof(int), hipHostMallocDefault));
         MP_HIP(hipHostMalloc(&hst_pinned, (size_t)nslots * sizeof(QueryStateHost), hipHostMallocDefault));
+        MP_HIP(hipHostMalloc(&hq, (size_t)nslots * sizeof(QueryDevHost), hipHostMallocDefault));
         MP_HIP(hipMemset(dst, 0, (size_t)nslots * sizeof(QueryStateHost)));
+        // every tree slot starts empty: NaN coordinates (0x7fc0dead) and parent -2
+        MP_HIP(hipMemsetD32((hipDeviceptr_t)trees, (int)0x7fc0deadu, (size_t)nslots * 2 * dof_max * cap));
+        MP_HIP(hipMemsetD32((hipDeviceptr_t)parents, -2, (size_t)nslots * 2 * cap));
+        MP_HIP(hipDeviceSynchronize());
     } catch (...) {
         this->~SlotPool();
         throw;
     }
-    hq.resize(nslots);
-    std::memset(hq.data(), 0, hq.size() * sizeof(QueryDevHost));
+    std::memset(hq, 0, (size_t)nslots * sizeof(QueryDevHost));
 }
 
 SlotPool::~SlotPool() {
@@ -51,6 +56,7 @@ SlotPool::~SlotPool() {
     if (dctl) hipFree(dctl);
     if (hpinned) hipHostFree(hpinned);
     if (hst_pinned) hipHostFree(hst_pinned);
+    if (hq) hipHostFree(hq);
     trees = nullptr;
     parents = nullptr;
     hpaths = nullptr;
@@ -61,6 +67,7 @@ SlotPool::~SlotPool() {
     dctl = nullptr;
     hpinned = nullptr;
     hst_pinned = nullptr;
+    hq = nullptr;
 }
 
 size_t SlotPool::bytes_per_slot() const { return slot_bytes(cap, dof_max); }
@@ -81,6 +88,7 @@ void SlotPool::fill_query(int slot, int robot_slot, int dof, const mp_scene* sce
     q.cap = cap;
     q.max_iters = max_iters;
     q.max_path = max_path;
+    q.dof_max = dof_max;
     for (int j = 0; j < dof; j++) {
         q.start[j] = start[j];
         q.goal[j] = goal[j];
@@ -99,12 +107,15 @@ PlanCtlHost SlotPool::ctl(float res, int max_step) const {
     return c;
 }
 
-int default_waves(hipFunction_t plan_fn) {
+int default_teams(hipFunction_t plan_fn, int team_waves) {
     int per_cu = 0;
-    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, plan_fn, 64, 0) != hipSuccess || per_cu <= 0)
-        per_cu = 4;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, plan_fn, 64 * team_waves, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
     return std::max(1, per_cu) * std::max(lib_num_cus(), 1);
 }
+
+int team_size(const mp_rrtc_params& p) { return p.team_waves > 0 ? p.team_waves : kDefaultTeamWaves; }
 
 namespace {
 std::mutex g_pool_mu;
@@ -135,6 +146,8 @@ void check_params(const mp_rrtc_params* p) {
     if (p->tree_capacity < 2 || p->tree_capacity > (1 << 22)) throw Error(MP_E_INVALID, "tree_capacity out of range");
     if (p->max_path < 2) throw Error(MP_E_INVALID, "max_path must be >= 2");
     if (p->waves < 0) throw Error(MP_E_INVALID, "waves must be >= 0");
+    if (p->team_waves != 0 && p->team_waves != 1 && p->team_waves != 2 && p->team_waves != 4 && p->team_waves != 8)
+        throw Error(MP_E_INVALID, "team_waves must be 0, 1, 2, 4 or 8");
 }
 
 double now_ms() {
@@ -148,9 +161,12 @@ void plan_release_workspaces() {
     g_pools.clear();
 }
 
-// Runs the queries already described in pool.hq[0..nq) to completion in one persistent launch.
-void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params& p, int waves, hipStream_t s) {
-    MP_HIP(hipMemcpyAsync(pool.dq, pool.hq.data(), (size_t)nq * sizeof(QueryDevHost), hipMemcpyHostToDevice, s));
+// Runs the queries already described in pool.hq[0..nq) to completion in one persistent launch
+// of `teams` workgroups of `team` waves.
+void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params& p, int team, int teams,
+                       hipStream_t s) {
+    const unsigned block = 64u * (unsigned)team;
+    MP_HIP(hipMemcpyAsync(pool.dq, pool.hq, (size_t)nq * sizeof(QueryDevHost), hipMemcpyHostToDevice, s));
     int* h = pool.hpinned;
     h[0] = 0;       // ticket
     h[1] = nq;      // running
@@ -164,8 +180,8 @@ void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params
     MP_HIP(hipMemcpyAsync(pool.dactive, h + 16, (size_t)nq * sizeof(int), hipMemcpyHostToDevice, s));
     PlanCtlHost ctl = pool.ctl(p.resolution, p.max_step);
     void* args[] = {&ctl};
-    launch(mod.fn("mp_plan_init"), (unsigned)nq, 64, s, args, "plan_init");
-    launch(mod.fn("mp_plan"), (unsigned)waves, 64, s, args, "plan");
+    launch(mod.fn("mp_plan_init"), (unsigned)nq, block, s, args, "plan_init");
+    launch(mod.fn("mp_plan"), (unsigned)teams, block, s, args, "plan");
     MP_HIP(hipMemcpyAsync(pool.hst_pinned, pool.dst, (size_t)nq * sizeof(QueryStateHost), hipMemcpyDeviceToHost, s));
     MP_HIP(hipStreamSynchronize(s));
 }
@@ -173,10 +189,11 @@ void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params
 void harvest(const SlotPool& pool, int slot, int dof, const QueryStateHost& st, mp_path* out) {
     out->status = st.status < 0 ? MP_PLAN_RUNNING : st.status;
     out->iterations = std::min(st.sample_ctr, pool.hq[slot].max_iters);
-    out->start_tree_size = std::min(st.pub[0], pool.cap);
-    out->goal_tree_size = std::min(st.pub[1], pool.cap);
+    out->start_tree_size = std::min(st.alloc[0], pool.cap);
+    out->goal_tree_size = std::min(st.alloc[1], pool.cap);
     out->configs_checked = (int64_t)st.checked;
     out->work_flops = (double)st.flops;
+    out->device_ms = st.t_done >= st.t_init ? (double)(st.t_done - st.t_init) * 1e-5 : 0.0;  // 100 MHz clock
     out->num_waypoints = 0;
     if (st.status == MP_PLAN_SOLVED) {
         if (st.path_len < 0) {
@@ -195,6 +212,70 @@ void harvest(const SlotPool& pool, int slot, int dof, const QueryStateHost& st, 
 }  // namespace mp
 
 namespace mp {
+
+// ------------------------------------------------------------------------------ launch shapes
+// The autotuner's choice of team size and grid per robot set and class (0: one query per
+// launch, 1: batches), kept in memory and in <cache dir>/tune_<hash>.txt ("class team waves").
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, std::pair<Shape, bool>> g_tuned;  // key#class -> (shape, looked up)
+
+std::string tune_file(const std::string& set_key) {
+    char h[32];
+    std::snprintf(h, sizeof(h), "%016llx", (unsigned long long)fnv1a(set_key));
+    return cache_dir() + "/tune_" + h + ".txt";
+}
+}  // namespace
+
+std::string robot_set_key(const std::vector<mp_robot*>& robots) {
+    std::vector<std::string> parts;
+    for (auto* r : robots) {
+        char h[32];
+        std::snprintf(h, sizeof(h), "%016llx", (unsigned long long)r->key);
+        parts.push_back(r->model.name + "#" + h);
+    }
+    std::sort(parts.begin(), parts.end());
+    parts.erase(std::unique(parts.begin(), parts.end()), parts.end());
+    std::string k;
+    for (auto& p : parts) k += p + ";";
+    return k;
+}
+
+Shape tuned_shape(const std::string& set_key, int cls) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    const std::string k = set_key + "@" + std::to_string(cls);
+    auto it = g_tuned.find(k);
+    if (it != g_tuned.end()) return it->second.first;
+    Shape sh;
+    if (FILE* f = std::fopen(tune_file(set_key).c_str(), "r")) {
+        int c, t, w;
+        while (std::fscanf(f, "%d %d %d", &c, &t, &w) == 3)
+            if (c == cls && (t == 1 || t == 2 || t == 4 || t == 8) && w >= 0) sh = Shape{t, w};
+        std::fclose(f);
+    }
+    g_tuned[k] = {sh, true};
+    return sh;
+}
+
+void store_shape(const std::string& set_key, int cls, Shape sh) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tuned[set_key + "@" + std::to_string(cls)] = {sh, true};
+    // rewrite the file with every class known for this set (best effort)
+    std::string lines;
+    for (int c = 0; c < 2; c++) {
+        auto it = g_tuned.find(set_key + "@" + std::to_string(c));
+        if (it != g_tuned.end() && it->second.first.team > 0)
+            lines += std::to_string(c) + " " + std::to_string(it->second.first.team) + " " +
+                     std::to_string(it->second.first.waves) + "\n";
+    }
+    const std::string path = tune_file(set_key), tmp = path + ".tmp";
+    if (FILE* f = std::fopen(tmp.c_str(), "w")) {
+        std::fputs(lines.c_str(), f);
+        std::fclose(f);
+        if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+    }
+}
+
 namespace {
 void solve_impl(const char* tag, int nq, mp_robot* const* robots, const mp_scene* const* scenes, const float* starts16,
                 const float* goals16, const mp_rrtc_params* params, mp_path* out, int default_wave_count) {
@@ -233,9 +314,20 @@ void solve_impl(const char* tag, int nq, mp_robot* const* robots, const mp_scene
     for (int k = 0; k < nq; k++)
         pool.fill_query(k, slot_of[k], robots[k]->model.dof, scenes[k], starts16 + 16 * k, goals16 + 16 * k,
                         params->seed + (uint64_t)k, params->max_iterations);
-    int waves = params->waves > 0 ? params->waves : (default_wave_count > 0 ? default_wave_count : default_waves(mod->fn("mp_plan")));
-    if (params->exclusive) waves = nq;
-    run_to_completion(*mod, pool, nq, *params, waves, s);
+    // launch shape: explicit parameters, else the tuned shape for this robot set and class, else
+    // the built-in defaults
+    int W = params->team_waves > 0 ? params->team_waves : kDefaultTeamWaves;
+    int waves = params->waves > 0 ? params->waves : default_wave_count;
+    if (params->team_waves == 0 && params->waves == 0) {
+        const Shape tuned = tuned_shape(robot_set_key(uniq), nq == 1 ? 0 : 1);
+        if (tuned.team > 0) {
+            W = tuned.team;
+            waves = tuned.waves;
+        }
+    }
+    int teams = waves > 0 ? std::max(1, waves / W) : default_teams(mod->fn("mp_plan"), W);
+    if (params->exclusive) teams = nq;
+    run_to_completion(*mod, pool, nq, *params, W, teams, s);
     for (int k = 0; k < nq; k++) harvest(pool, k, robots[k]->model.dof, pool.hst_pinned[k], &out[k]);
     double lat = now_ms() - t0;
     for (int k = 0; k < nq; k++) out[k].latency_ms = lat;
@@ -255,6 +347,7 @@ void mp_rrtc_default_params(mp_rrtc_params* p) {
     p->max_path = 4096;
     p->seed = 0;
     p->exclusive = 0;
+    p->team_waves = 0;
 }
 
 int mp_rrtc_solve(mp_robot* robot, const mp_scene* scene, const float* start, const float* goal,
@@ -272,6 +365,62 @@ int mp_rrtc_solve(mp_robot* robot, const mp_scene* scene, const float* start, co
 int mp_rrtc_solve_batch(int nq, mp_robot* const* robots, const mp_scene* const* scenes, const float* starts16,
                         const float* goals16, const mp_rrtc_params* params, mp_path* out) {
     return mp::guard_call([&] { mp::solve_impl("batch", nq, robots, scenes, starts16, goals16, params, out, 0); });
+}
+
+int mp_rrtc_autotune(int nq, mp_robot* const* robots, const mp_scene* const* scenes, const float* starts16,
+                     const float* goals16, const mp_rrtc_params* params, int per_query, int reps,
+                     mp_tune_result* results, int cap, int* n_results, mp_rrtc_params* best) {
+    return mp::guard_call([&] {
+        if (!params || !best || nq < 1 || reps < 1 || !robots || !scenes || !starts16 || !goals16)
+            throw Error(MP_E_INVALID, "bad autotune arguments");
+        if (params->exclusive) throw Error(MP_E_INVALID, "autotune the shared mode (exclusive = 0)");
+        for (int k = 0; k < nq; k++)
+            if (!robots[k] || !scenes[k]) throw Error(MP_E_INVALID, "null robot or scene in the sample");
+        const std::string key = mp::robot_set_key(std::vector<mp_robot*>(robots, robots + nq));
+        const int cls = per_query ? 0 : 1;
+        // candidates: waves per team x total waves (0 = every team the device keeps resident)
+        const int team_sizes[] = {2, 4, 8};
+        const int plain_waves[] = {256, 512, 1024, 2048};
+        const int batch_waves[] = {0, 1024, 2048, 4096};
+        std::vector<mp_path> out((size_t)nq);
+        int bt = 0, bw = 0, n = 0;
+        double bms = 1e300;
+        for (int t : team_sizes) {
+            for (int i = 0; i < 4; i++) {
+                mp_rrtc_params p = *params;
+                p.team_waves = t;
+                p.waves = per_query ? plain_waves[i] : batch_waves[i];
+                double ms = 0.0;
+                for (int r = 0; r < reps; r++) {
+                    p.seed = params->seed + 7919ull * (uint64_t)r;
+                    for (auto& o : out) std::memset(&o, 0, sizeof(o));
+                    if (per_query) {
+                        for (int k = 0; k < nq; k++) {
+                            mp::solve_impl("plain", 1, robots + k, scenes + k, starts16 + 16 * k, goals16 + 16 * k, &p,
+                                           &out[k], 512);
+                            ms += mp_last_kernel_ms("plan");
+                        }
+                    } else {
+                        mp::solve_impl("batch", nq, robots, scenes, starts16, goals16, &p, out.data(), 0);
+                        ms += mp_last_kernel_ms("plan");
+                    }
+                }
+                ms /= reps;
+                if (results && n < cap) results[n] = mp_tune_result{t, p.waves, ms};
+                n++;
+                if (ms < bms) {
+                    bms = ms;
+                    bt = t;
+                    bw = p.waves;
+                }
+            }
+        }
+        if (n_results) *n_results = n;
+        mp::store_shape(key, cls, mp::Shape{bt, bw});
+        *best = *params;
+        best->team_waves = bt;
+        best->waves = bw;
+    });
 }
 
 }  // extern "C"

@@ -56,7 +56,7 @@ struct mp_server {
     std::vector<int> robot_slot;  // robot index -> the module's robot slot
     std::shared_ptr<mp::Module> mod;
     std::unique_ptr<mp::SlotPool> pool;
-    int nslots = 0, waves = 0, budget = 0;
+    int nslots = 0, teams = 0, team = 1, budget = 0;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
@@ -103,8 +103,8 @@ void mp_server::step(const std::vector<int>& list, int n_new) {
             MP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             MP_HIP(hipMemcpyAsync(pool->dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, stream));
             MP_HIP(hipMemcpyAsync(pool->dactive, h + 16, (size_t)nslots * sizeof(int), hipMemcpyHostToDevice, stream));
-            MP_HIP(hipModuleLaunchKernel(mod->fn("mp_plan_init"), nslots, 1, 1, 64, 1, 1, 0, stream, args, nullptr));
-            MP_HIP(hipModuleLaunchKernel(mod->fn("mp_plan"), waves, 1, 1, 64, 1, 1, 0, stream, args, nullptr));
+            MP_HIP(hipModuleLaunchKernel(mod->fn("mp_plan_init"), nslots, 1, 1, 64 * team, 1, 1, 0, stream, args, nullptr));
+            MP_HIP(hipModuleLaunchKernel(mod->fn("mp_plan"), teams, 1, 1, 64 * team, 1, 1, 0, stream, args, nullptr));
             MP_HIP(hipMemcpyAsync(pool->hst_pinned, pool->dst, (size_t)nslots * sizeof(mp::QueryStateHost),
                                   hipMemcpyDeviceToHost, stream));
             MP_HIP(hipStreamEndCapture(stream, &graph_def));
@@ -116,8 +116,8 @@ void mp_server::step(const std::vector<int>& list, int n_new) {
         void* args[] = {&ctl};
         MP_HIP(hipMemcpyAsync(pool->dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, stream));
         MP_HIP(hipMemcpyAsync(pool->dactive, h + 16, list.size() * sizeof(int), hipMemcpyHostToDevice, stream));
-        if (n_new > 0) mp::launch(mod->fn("mp_plan_init"), (unsigned)n_new, 64, stream, args, "plan_init");
-        mp::launch(mod->fn("mp_plan"), (unsigned)waves, 64, stream, args, "plan");
+        if (n_new > 0) mp::launch(mod->fn("mp_plan_init"), (unsigned)n_new, 64u * team, stream, args, "plan_init");
+        mp::launch(mod->fn("mp_plan"), (unsigned)teams, 64u * team, stream, args, "plan");
         MP_HIP(hipMemcpyAsync(pool->hst_pinned, pool->dst, (size_t)nslots * sizeof(mp::QueryStateHost),
                               hipMemcpyDeviceToHost, stream));
     }
@@ -231,9 +231,20 @@ int mp_server_create(const mp_server_params* params, mp_server** out) {
         s->nslots = (int)std::min<size_t>((size_t)params->max_active, fit);
         if (s->nslots < 1) throw Error(MP_E_CAPACITY, "tree memory too small for one query");
         s->pool = std::make_unique<mp::SlotPool>(s->nslots, params->tree_capacity, dof_max, p.max_path);
-        s->waves = params->waves > 0 ? params->waves : mp::default_waves(s->mod->fn("mp_plan"));
-        if (p.exclusive) s->waves = s->nslots;
-        s->budget = params->step_iterations > 0 ? params->step_iterations : 2 * s->waves;
+        if (p.team_waves != 0 && p.team_waves != 1 && p.team_waves != 2 && p.team_waves != 4 && p.team_waves != 8)
+            throw Error(MP_E_INVALID, "team_waves must be 0, 1, 2, 4 or 8");
+        s->team = mp::team_size(p);
+        int waves = params->waves;
+        if (p.team_waves == 0 && params->waves == 0) {  // the tuned batch shape of this robot set
+            const mp::Shape tuned = mp::tuned_shape(mp::robot_set_key(s->robots), 1);
+            if (tuned.team > 0) {
+                s->team = tuned.team;
+                waves = tuned.waves;
+            }
+        }
+        s->teams = waves > 0 ? std::max(1, waves / s->team) : mp::default_teams(s->mod->fn("mp_plan"), s->team);
+        if (p.exclusive) s->teams = s->nslots;
+        s->budget = params->step_iterations > 0 ? params->step_iterations : 2 * s->teams;
         MP_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         for (int k = s->nslots - 1; k >= 0; k--) s->free_slots.push_back(k);
         mp_server* raw = s.release();

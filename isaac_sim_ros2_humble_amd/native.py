@@ -30,13 +30,18 @@ class RobotDesc(C.Structure):
 class RRTCParams(C.Structure):
     _fields_ = [("resolution", C.c_float), ("max_step", C.c_int), ("max_iterations", C.c_int),
                 ("tree_capacity", C.c_int), ("waves", C.c_int), ("max_path", C.c_int), ("seed", C.c_uint64),
-                ("exclusive", C.c_int)]
+                ("exclusive", C.c_int), ("team_waves", C.c_int)]
 
 
 class Path(C.Structure):
     _fields_ = [("status", C.c_int), ("num_waypoints", C.c_int), ("waypoints", C.c_void_p), ("capacity", C.c_int),
                 ("iterations", C.c_int), ("start_tree_size", C.c_int), ("goal_tree_size", C.c_int),
-                ("configs_checked", C.c_int64), ("latency_ms", C.c_double), ("work_flops", C.c_double)]
+                ("configs_checked", C.c_int64), ("latency_ms", C.c_double), ("work_flops", C.c_double),
+                ("device_ms", C.c_double)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("team_waves", C.c_int), ("waves", C.c_int), ("kernel_ms", C.c_double)]
 
 
 class ServerParams(C.Structure):
@@ -88,6 +93,8 @@ _SIGS = {
     "mp_rrtc_default_params": (None, [C.POINTER(RRTCParams)]),
     "mp_rrtc_solve": (C.c_int, [P, P, P, P, C.POINTER(RRTCParams), C.POINTER(Path)]),
     "mp_rrtc_solve_batch": (C.c_int, [C.c_int, P, P, P, P, C.POINTER(RRTCParams), P]),
+    "mp_rrtc_autotune": (C.c_int, [C.c_int, P, P, P, P, C.POINTER(RRTCParams), C.c_int, C.c_int, P, C.c_int,
+                                   C.POINTER(C.c_int), C.POINTER(RRTCParams)]),
     "mp_server_create": (C.c_int, [C.POINTER(ServerParams), C.POINTER(C.c_void_p)]),
     "mp_server_submit": (C.c_int, [P, C.c_int, P, P, P, C.c_uint64, C.POINTER(C.c_int64)]),
     "mp_server_poll": (C.c_int, [P, C.c_int64, C.POINTER(Path)]),
@@ -353,13 +360,14 @@ def _path_result(pth, wp, dof):
     n = max(pth.num_waypoints, 0) if pth.status == 1 else 0
     return {"status": PLAN_STATUS.get(pth.status, str(pth.status)), "path": wp[:n].reshape(-1, dof).copy(),
             "iterations": pth.iterations, "start_tree": pth.start_tree_size, "goal_tree": pth.goal_tree_size,
-            "configs_checked": pth.configs_checked, "latency_ms": pth.latency_ms, "work_flops": pth.work_flops}
+            "configs_checked": pth.configs_checked, "latency_ms": pth.latency_ms, "work_flops": pth.work_flops,
+            "device_ms": pth.device_ms}
 
 
 def rrtc_solve(robot, scene, start, goal, params=None, max_path=4096):
     params = params or default_params()
     wp = np.empty((params.max_path, robot.dof), np.float32)
-    pth = Path(0, 0, _p(wp), params.max_path, 0, 0, 0, 0, 0.0, 0.0)
+    pth = Path(0, 0, _p(wp), params.max_path, 0, 0, 0, 0, 0.0, 0.0, 0.0)
     s = np.ascontiguousarray(start, np.float32)
     g = np.ascontiguousarray(goal, np.float32)
     check(lib().mp_rrtc_solve(robot.h, scene.h, _p(s), _p(g), C.byref(params), C.byref(pth)), "mp_rrtc_solve")
@@ -381,12 +389,33 @@ def rrtc_solve_batch(robots, scenes, starts, goals, params=None):
         g16[k, :d] = goals[k]
         wp = block[k, :params.max_path * d].reshape(params.max_path, d)
         wps.append(wp)
-        paths[k] = Path(0, 0, _p(wp), params.max_path, 0, 0, 0, 0, 0.0, 0.0)
+        paths[k] = Path(0, 0, _p(wp), params.max_path, 0, 0, 0, 0, 0.0, 0.0, 0.0)
     rh = (C.c_void_p * nq)(*[r.h for r in robots])
     sh = (C.c_void_p * nq)(*[s.h for s in scenes])
     check(lib().mp_rrtc_solve_batch(nq, C.cast(rh, C.c_void_p), C.cast(sh, C.c_void_p), _p(s16), _p(g16),
                                     C.byref(params), C.cast(paths, C.c_void_p)), "mp_rrtc_solve_batch")
     return [_path_result(paths[k], wps[k], robots[k].dof) for k in range(nq)]
+
+
+def rrtc_autotune(robots, scenes, starts, goals, params=None, per_query=False, reps=2):
+    """Tunes the planning launch shape on the sample (mp_rrtc_autotune); returns the best params
+    and the candidates as dicts (team_waves, waves, kernel_ms)."""
+    params = params or default_params()
+    nq = len(robots)
+    s16 = np.zeros((nq, 16), np.float32)
+    g16 = np.zeros((nq, 16), np.float32)
+    for k in range(nq):
+        s16[k, :robots[k].dof] = starts[k]
+        g16[k, :robots[k].dof] = goals[k]
+    rh = (C.c_void_p * nq)(*[r.h for r in robots])
+    sh = (C.c_void_p * nq)(*[s.h for s in scenes])
+    res = (TuneResult * 64)()
+    n = C.c_int()
+    best = RRTCParams()
+    check(lib().mp_rrtc_autotune(nq, C.cast(rh, C.c_void_p), C.cast(sh, C.c_void_p), _p(s16), _p(g16), C.byref(params),
+                                 1 if per_query else 0, reps, C.cast(res, C.c_void_p), 64, C.byref(n), C.byref(best)),
+          "mp_rrtc_autotune")
+    return best, [{"team_waves": r.team_waves, "waves": r.waves, "kernel_ms": r.kernel_ms} for r in res[:min(n.value, 64)]]
 
 
 class Server:
@@ -415,7 +444,7 @@ class Server:
 
     def poll(self, ticket, dof):
         wp = np.zeros((self.max_path, dof), np.float32)
-        pth = Path(0, 0, _p(wp), self.max_path, 0, 0, 0, 0, 0.0, 0.0)
+        pth = Path(0, 0, _p(wp), self.max_path, 0, 0, 0, 0, 0.0, 0.0, 0.0)
         rc = lib().mp_server_poll(self.h, ticket, C.byref(pth))
         if rc < 0:
             check(rc, "mp_server_poll")

@@ -127,6 +127,35 @@ def test_exclusive_mode_deterministic_and_batch_equals_solo(fr3, fetch):
             _assert_valid(m, sc, solo[k], s, g)
 
 
+def test_exclusive_mode_team_size_invariant(fr3, fetch):
+    """One team per query: the nearest-neighbour ties go to the lowest index and a check's
+    verdict on its deciding lanes is exact for any split of the work across the team's waves,
+    so the planner's decisions, and the paths, are the same for 1, 2, 4 and 8 waves per team."""
+    rng = np.random.default_rng(21)
+    queries = []
+    for k in range(6):
+        m = fr3 if k % 2 == 0 else fetch
+        scene = workloads.random_scene(rng, workloads.log_uniform_count(rng, 5, 120), *workloads.workspace(m))
+        pair = workloads.sample_pair(m, scene, rng, gpu_check, gpu_edge, want_free_line=(k % 3 == 0))
+        assert pair is not None
+        queries.append((m, scene, pair[0], pair[1]))
+    robots = [_robot(m) for m, _, _, _ in queries]
+    scenes = [native.Scene(sc) for _, sc, _, _ in queries]
+    starts, goals = [s for _, _, s, _ in queries], [g for _, _, _, g in queries]
+    ref = None
+    for team in (1, 2, 4, 8):
+        res = native.rrtc_solve_batch(robots, scenes, starts, goals,
+                                      native.default_params(exclusive=1, seed=5, team_waves=team))
+        got = [(r["status"], r["path"], r["iterations"], r["start_tree"], r["goal_tree"], r["configs_checked"])
+               for r in res]
+        if ref is None:
+            ref = got
+            assert sum(r[0] == "solved" for r in got) >= 4
+            continue
+        for a, b in zip(ref, got):
+            assert a[0] == b[0] and np.array_equal(a[1], b[1]) and a[2:] == b[2:], (team, a[0], b[0], a[2:], b[2:])
+
+
 def test_batch_shared_mixed_robots(fr3, fetch):
     rng = np.random.default_rng(9)
     robots, scenes, starts, goals, models = [], [], [], [], []
@@ -146,3 +175,24 @@ def test_batch_shared_mixed_robots(fr3, fetch):
         if r["status"] == "solved":
             valid += _assert_valid(m, sc, r, s, g)
     assert valid >= 0.9 * len(solved)
+
+
+def test_autotune_picks_fastest_candidate_and_is_used(fr3, fetch):
+    rng = np.random.default_rng(31)
+    robots, scenes, starts, goals = [], [], [], []
+    for k in range(12):
+        m = fr3 if k % 2 == 0 else fetch
+        scene = workloads.random_scene(rng, workloads.log_uniform_count(rng, 5, 60), *workloads.workspace(m))
+        pair = workloads.sample_pair(m, scene, rng, gpu_check, gpu_edge, want_free_line=(k % 2 == 0))
+        if pair is None:
+            continue
+        robots.append(_robot(m)); scenes.append(native.Scene(scene)); starts.append(pair[0]); goals.append(pair[1])
+    best, cands = native.rrtc_autotune(robots, scenes, starts, goals, native.default_params(seed=4), reps=1)
+    assert len(cands) == 12
+    assert {c["team_waves"] for c in cands} == {2, 4, 8}
+    fastest = min(cands, key=lambda c: c["kernel_ms"])
+    assert (best.team_waves, best.waves) == (fastest["team_waves"], fastest["waves"])
+    assert all(c["kernel_ms"] > 0 for c in cands)
+    # later batches with team_waves = waves = 0 run with the stored shape
+    res = native.rrtc_solve_batch(robots, scenes, starts, goals, native.default_params(seed=9))
+    assert sum(r["status"] == "solved" for r in res) >= 0.8 * len(res)

@@ -43,6 +43,19 @@ def test_generated_source_structure(fr3, fetch):
     assert fk.count("mp_sincosf(") == 7
 
 
+def test_generated_checks_split_across_a_team(fr3):
+    """Every self-collision block is guarded by its team share and every obstacle loop strides by
+    the team size, so a team of W waves covers each block and obstacle exactly once."""
+    src = native.generate_source([fr3])
+    r0 = src[src.index("struct Robot0"):]
+    hit = r0[r0.index("self_hit"):r0.index("self_cost")]
+    blocks = hit.count("T.mine(")
+    assert blocks == len(set(int(x.split(")")[0]) for x in hit.split("T.mine(")[1:]))
+    assert [int(x.split(")")[0]) for x in hit.split("T.mine(")[1:]] == list(range(blocks))
+    env = r0[r0.index("env_hit"):r0.index("env_cost")]
+    assert env.count("o += os)") == 4 and "const int o0 = mp_uniform(T.w), os = mp_uniform(T.W);" in env
+
+
 def test_generated_source_is_deterministic_and_robot_specific(fr3):
     a = native.generate_source([fr3])
     assert a == native.generate_source([fr3])
