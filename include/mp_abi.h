@@ -186,7 +186,7 @@ typedef struct {
     uint64_t seed;         /* sample stream seed; query k uses seed + k */
     int exclusive;         /* 1: one team grows each query (deterministic per seed, identical
                               alone and in a batch); 0: all teams share all queries */
-    int team_waves;        /* waves per team (1, 2, 4 or 8; 0 = tuned default): the waves of a
+    int team_waves;        /* waves per team (1, 2 or 4; 0 = tuned default): the waves of a
                               team share one iteration's nearest-neighbour scans and collision
                               checks (threads per tree iteration = 64 * team_waves) */
 } mp_rrtc_params;
@@ -228,7 +228,7 @@ int mp_rrtc_solve_batch(int nq, mp_robot* const* robots, const mp_scene* const* 
 
 /* Autotuner of the planning launch shape (no counterpart in the reference; CPRRTC fixes one
  * block of threads per worker).  Solves the sample queries once per candidate shape and seed
- * (team_waves 2, 4, 8 x total waves; per_query = 1 tunes the one-query-per-launch shape by
+ * (team_waves 1, 2, 4 x total waves; per_query = 1 tunes the one-query-per-launch shape by
  * solving the sample one query at a time, 0 tunes the batch shape with the sample as one batch),
  * scores each by its mean planning-kernel time, and returns the fastest in *best (params with
  * team_waves and waves set).  The choice is kept for this robot set and class, in memory and in

@@ -146,8 +146,8 @@ void check_params(const mp_rrtc_params* p) {
     if (p->tree_capacity < 2 || p->tree_capacity > (1 << 22)) throw Error(MP_E_INVALID, "tree_capacity out of range");
     if (p->max_path < 2) throw Error(MP_E_INVALID, "max_path must be >= 2");
     if (p->waves < 0) throw Error(MP_E_INVALID, "waves must be >= 0");
-    if (p->team_waves != 0 && p->team_waves != 1 && p->team_waves != 2 && p->team_waves != 4 && p->team_waves != 8)
-        throw Error(MP_E_INVALID, "team_waves must be 0, 1, 2, 4 or 8");
+    if (p->team_waves != 0 && p->team_waves != 1 && p->team_waves != 2 && p->team_waves != 4)
+        throw Error(MP_E_INVALID, "team_waves must be 0, 1, 2 or 4");
 }
 
 double now_ms() {
@@ -250,7 +250,7 @@ Shape tuned_shape(const std::string& set_key, int cls) {
     if (FILE* f = std::fopen(tune_file(set_key).c_str(), "r")) {
         int c, t, w;
         while (std::fscanf(f, "%d %d %d", &c, &t, &w) == 3)
-            if (c == cls && (t == 1 || t == 2 || t == 4 || t == 8) && w >= 0) sh = Shape{t, w};
+            if (c == cls && (t == 1 || t == 2 || t == 4) && w >= 0) sh = Shape{t, w};
         std::fclose(f);
     }
     g_tuned[k] = {sh, true};
@@ -379,7 +379,7 @@ int mp_rrtc_autotune(int nq, mp_robot* const* robots, const mp_scene* const* sce
         const std::string key = mp::robot_set_key(std::vector<mp_robot*>(robots, robots + nq));
         const int cls = per_query ? 0 : 1;
         // candidates: waves per team x total waves (0 = every team the device keeps resident)
-        const int team_sizes[] = {2, 4, 8};
+        const int team_sizes[] = {1, 2, 4};
         const int plain_waves[] = {256, 512, 1024, 2048};
         const int batch_waves[] = {0, 1024, 2048, 4096};
         std::vector<mp_path> out((size_t)nq);

@@ -231,8 +231,8 @@ int mp_server_create(const mp_server_params* params, mp_server** out) {
         s->nslots = (int)std::min<size_t>((size_t)params->max_active, fit);
         if (s->nslots < 1) throw Error(MP_E_CAPACITY, "tree memory too small for one query");
         s->pool = std::make_unique<mp::SlotPool>(s->nslots, params->tree_capacity, dof_max, p.max_path);
-        if (p.team_waves != 0 && p.team_waves != 1 && p.team_waves != 2 && p.team_waves != 4 && p.team_waves != 8)
-            throw Error(MP_E_INVALID, "team_waves must be 0, 1, 2, 4 or 8");
+        if (p.team_waves != 0 && p.team_waves != 1 && p.team_waves != 2 && p.team_waves != 4)
+            throw Error(MP_E_INVALID, "team_waves must be 0, 1, 2 or 4");
         s->team = mp::team_size(p);
         int waves = params->waves;
         if (p.team_waves == 0 && params->waves == 0) {  // the tuned batch shape of this robot set

@@ -374,27 +374,52 @@ def rrtc_solve(robot, scene, start, goal, params=None, max_path=4096):
     return _path_result(pth, wp, robot.dof)
 
 
+# numpy view of an mp_path array (layout checked against the ctypes structure at import)
+PATH_DTYPE = np.dtype([("status", "<i4"), ("num_waypoints", "<i4"), ("waypoints", "<u8"), ("capacity", "<i4"),
+                       ("iterations", "<i4"), ("start_tree_size", "<i4"), ("goal_tree_size", "<i4"),
+                       ("configs_checked", "<i8"), ("latency_ms", "<f8"), ("work_flops", "<f8"), ("device_ms", "<f8")])
+assert PATH_DTYPE.itemsize == C.sizeof(Path) and all(
+    PATH_DTYPE.fields[n][1] == getattr(Path, n).offset for n, _ in Path._fields_)
+
+_WP_BLOCK = [np.empty((0,), np.float32)]  # waypoint storage reused across batches (pages stay mapped)
+
+
+def _waypoint_block(n):
+    if _WP_BLOCK[0].size < n:
+        _WP_BLOCK[0] = np.zeros(n, np.float32)
+    return _WP_BLOCK[0]
+
+
 def rrtc_solve_batch(robots, scenes, starts, goals, params=None):
     params = params or default_params()
     nq = len(robots)
+    dof = np.array([r.dof for r in robots])
     s16 = np.zeros((nq, 16), np.float32)
     g16 = np.zeros((nq, 16), np.float32)
-    # one uninitialised block for every query's waypoints (only written rows are touched)
-    block = np.empty((nq, params.max_path * 16), np.float32)
-    wps = []
-    paths = (Path * nq)()
     for k in range(nq):
-        d = robots[k].dof
-        s16[k, :d] = starts[k]
-        g16[k, :d] = goals[k]
-        wp = block[k, :params.max_path * d].reshape(params.max_path, d)
-        wps.append(wp)
-        paths[k] = Path(0, 0, _p(wp), params.max_path, 0, 0, 0, 0, 0.0, 0.0, 0.0)
+        s16[k, :dof[k]] = starts[k]
+        g16[k, :dof[k]] = goals[k]
+    stride = params.max_path * 16
+    block = _waypoint_block(nq * stride)
+    paths = np.zeros(nq, PATH_DTYPE)
+    paths["waypoints"] = block.ctypes.data + np.arange(nq, dtype=np.uint64) * np.uint64(stride * 4)
+    paths["capacity"] = params.max_path
     rh = (C.c_void_p * nq)(*[r.h for r in robots])
     sh = (C.c_void_p * nq)(*[s.h for s in scenes])
     check(lib().mp_rrtc_solve_batch(nq, C.cast(rh, C.c_void_p), C.cast(sh, C.c_void_p), _p(s16), _p(g16),
-                                    C.byref(params), C.cast(paths, C.c_void_p)), "mp_rrtc_solve_batch")
-    return [_path_result(paths[k], wps[k], robots[k].dof) for k in range(nq)]
+                                    C.byref(params), paths.ctypes.data_as(C.c_void_p)), "mp_rrtc_solve_batch")
+    out = []
+    for k in range(nq):
+        r = paths[k]
+        st = int(r["status"])
+        n = max(int(r["num_waypoints"]), 0) if st == 1 else 0
+        d = int(dof[k])
+        out.append({"status": PLAN_STATUS.get(st, str(st)), "path": block[k * stride:k * stride + n * d].reshape(n, d).copy(),
+                    "iterations": int(r["iterations"]), "start_tree": int(r["start_tree_size"]),
+                    "goal_tree": int(r["goal_tree_size"]), "configs_checked": int(r["configs_checked"]),
+                    "latency_ms": float(r["latency_ms"]), "work_flops": float(r["work_flops"]),
+                    "device_ms": float(r["device_ms"])})
+    return out
 
 
 def rrtc_autotune(robots, scenes, starts, goals, params=None, per_query=False, reps=2):
