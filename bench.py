@@ -35,6 +35,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel, path=os.path.join(REPO, "profiles", "traffic.txt")):
+    """HBM bytes per launch (fetch + write) of `kernel` from the last rocprofv3 --pmc runs
+    (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel factors), or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                w = line.split()
+                if len(w) >= 3 and w[0] == kernel:
+                    return float(w[1]) + float(w[2])
+    except OSError:
+        pass
+    return None
+
+
 def build_c4(robots, models, n, seed):
     def check(m, scene, q):
         return native.check_states(robots[m.name], native.Scene(scene), q)
@@ -131,8 +145,48 @@ def cpu_baseline(models, queries, budget_s, threads):
                                                                       solved, dt)}
 
 
-def cpu_checker_c2(model, threads, n=1 << 16):
-    """The CPU FP32 FK + collision checker (oracle) on the first n of the 2^20 C2 configurations."""
+def bench_c1(robot, model, seeds, threads):
+    """C1: FR3 with 20 yaw-only boxes, one query per seed.  CPU: the oracle's RRT-Connect (OMPL
+    semantics, range 0.3 rad, resolution 0.02 rad), one thread per query, `threads` queries at a
+    time; GPU: the plain one-query-per-launch planner on the same queries."""
+    import oracle
+    scene = workloads.c1_scene(seed=0)
+    nscene = native.Scene(scene)
+
+    def check(m, sc, q):
+        return native.check_states(robot, nscene, q)
+
+    def edge(m, sc, a, b):
+        ok, _ = native.check_edges(robot, nscene, a, b, 0.02)
+        return ok
+    pairs = [workloads.sample_pair(model, scene, np.random.default_rng(k), check, edge, want_free_line=False)
+             for k in range(seeds)]
+    orc, osc = oracle.OracleRobot(model), oracle.OracleScene(scene)
+
+    def cpu_one(k):
+        t0 = time.perf_counter()
+        _, st = oracle.rrtconnect(orc, osc, pairs[k][0], pairs[k][1], range_=0.3, resolution=0.02, seed=k,
+                                  time_limit=10.0)
+        return time.perf_counter() - t0, st.solved == 1
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        cpu = list(ex.map(cpu_one, range(seeds)))
+    wall = time.perf_counter() - t0
+    gpu = []
+    for k in range(seeds):
+        r = native.rrtc_solve(robot, nscene, pairs[k][0], pairs[k][1], native.default_params(seed=k))
+        gpu.append((r["latency_ms"] / 1e3, r["status"] == "solved"))
+    ct = np.array([c[0] for c in cpu]) * 1e3
+    gt = np.array([g[0] for g in gpu]) * 1e3
+    return {"seeds": seeds, "cpu_seed0_ms": float(ct[0]), "cpu_median_ms": float(np.median(ct)),
+            "cpu_solved": int(sum(c[1] for c in cpu)), "cpu_threads": threads,
+            "cpu_qps": float(sum(c[1] for c in cpu) / wall), "gpu_seed0_ms": float(gt[0]),
+            "gpu_median_ms": float(np.median(gt)), "gpu_solved": int(sum(g[1] for g in gpu)),
+            "median_speedup": float(np.median(ct) / np.median(gt))}
+
+
+def cpu_checker_c2(model, threads, n=1 << 20):
+    """The CPU FP32 FK + collision checker (oracle) on the C2 configurations."""
     import oracle
     scene = workloads.c2_scene(seed=1)
     q = workloads.c2_configs(model, 1 << 20, seed=1)[:n]
@@ -141,7 +195,7 @@ def cpu_checker_c2(model, threads, n=1 << 16):
     oracle.check(orc, osc, q, nthreads=threads)
     dt = time.perf_counter() - t0
     return {"value": n / dt, "unit": "configs/s", "cores": threads, "kind": "port",
-            "sample": "first %d of the 2^20 C2 configurations (FR3, 64 obstacles), verdict mode" % n}
+            "sample": "%d C2 configurations (FR3, 64 obstacles), verdict mode" % n}
 
 
 def bench_c3(robot, model, seeds):
@@ -223,9 +277,12 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--rate", type=float, nargs="*", default=[100.0, 1000.0, 5000.0])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--c3-seeds", type=int, default=100)
+    ap.add_argument("--c3-seeds", type=int, default=1000)
+    ap.add_argument("--c1-seeds", type=int, default=100)
     ap.add_argument("--tune-queries", type=int, default=128)
-    ap.add_argument("--skip", nargs="*", default=[], help="parts to skip: tune plain serve c2 c3 mbm cpu")
+    ap.add_argument("--team-waves", type=int, default=0, help="planning launch shape (0: tuned / default)")
+    ap.add_argument("--waves", type=int, default=0, help="planning grid in waves (0: tuned / resident)")
+    ap.add_argument("--skip", nargs="*", default=[], help="parts to skip: tune plain serve c2 c3 mbm cpu (cpu also skips c1)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: queries shard across GPUs with no exchange, so the "
@@ -258,7 +315,7 @@ def main():
             tune[name] = {"team_waves": best.team_waves, "waves": best.waves, "calibration_queries": n,
                           "candidates": cands}
             log("autotune %s: team_waves %d waves %d" % (name, best.team_waves, best.waves))
-    params = native.default_params(seed=0)
+    params = native.default_params(seed=0, team_waves=a.team_waves, waves=a.waves)
 
     # batched: warmup, then exactly K timed steps
     for _ in range(a.warmup):
@@ -283,6 +340,7 @@ def main():
     # one more (untimed) step with the planner's phase profile on: where the waves spend their time
     native.plan_profile(enable=True)
     params.seed = 999
+    shape = {"team_waves": a.team_waves, "waves": a.waves} if a.team_waves or a.waves else tune.get("batch")
     run_batch(rob_list, scenes, starts, goals, params)
     prof = native.plan_profile(enable=False)
     wave = max(prof["wave"], 1)
@@ -304,8 +362,9 @@ def main():
                                                        float(np.percentile(dev_ms, 99)), float(dev_ms.max())],
                        "phase_fraction_of_wave_time": phase},
            "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                        "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                        "kernel": "mp_plan (batched)", "flops_per_launch": float(np.mean(flops))},
+                        "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic("mp_plan"),
+                        "kernel": "mp_plan (batched)", "flops_per_launch": float(np.mean(flops)),
+                        "launch_shape": {"team_waves": shape["team_waves"], "waves": shape["waves"]} if shape else None},
            "autotune": tune}
 
     if "plain" not in a.skip:
@@ -349,6 +408,9 @@ def main():
         log("cpu:", out["cpu_baseline"])
         out["cpu_checker_c2"] = cpu_checker_c2(models[0], threads)
         log("cpu c2:", out["cpu_checker_c2"])
+        if a.c1_seeds > 0:
+            out["c1"] = bench_c1(robots["fr3"], models[0], a.c1_seeds, threads)
+            log("c1:", out["c1"])
     print(json.dumps(out), flush=True)
 
 
