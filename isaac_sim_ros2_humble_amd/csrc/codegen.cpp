@@ -10,6 +10,7 @@
 // generated code evaluates the same FP32 rounding sequence as the dense rule set:
 //   dot3(a,b) = fma(a0,b0, fma(a1,b1, a2*b2)),  row(a,b,t) = fma(a0,b0, fma(a1,b1, fma(a2,b2,t)))
 //   fixed T = P o O;  revolute T = (P o O) o Rot(axis,q);  prismatic T = P o (O.R, O.R*(axis*q)+O.t)
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <array>
@@ -63,10 +64,11 @@ class Emitter {
     std::ostringstream os;
     int n = 0;
     std::string ind = "        ";
+    std::string pfx = "t";  // temporary names: pfx + counter
 
     std::string ref(const V& v) const { return v.k ? lit(v.c) : v.e; }
     V tmp(const std::string& expr) {
-        std::string name = "t" + std::to_string(n++);
+        std::string name = pfx + std::to_string(n++);
         os << ind << "const float " << name << " = " << expr << ";\n";
         return V{false, 0.0f, name};
     }
@@ -431,6 +433,53 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
         return X.os.str();
     };
 
+    // ---------------------------------------------------------------- link groups
+    // Obstacle-level culling: the sphere links are split into contiguous groups; per lane, a
+    // group's bounding sphere is centred on its middle link's bounding-sphere centre with radius
+    // max(|b_k - c| + R_k) + 2 margins.  An obstacle whose bounding sphere misses every lane's
+    // group sphere cannot reach any link of the group (triangle inequality), so the group's link
+    // tests are skipped.
+    std::vector<std::vector<int>> groups;
+    {
+        const int G = NL <= 2 ? 1 : std::min(4, (NL + 2) / 3);
+        for (int g = 0; g < G; g++) {
+            std::vector<int> v;
+            for (int k = g * NL / G; k < (g + 1) * NL / G; k++) v.push_back(k);
+            if (!v.empty()) groups.push_back(v);
+        }
+    }
+    Emitter GP;
+    GP.pfx = "gp";
+    std::vector<std::array<V, 4>> gsph;
+    for (const auto& g : groups) {
+        const int km = g[g.size() / 2];
+        V c[3] = {FV[km][12], FV[km][13], FV[km][14]};
+        V r = K(0.0f);
+        bool first = true;
+        for (int k : g) {
+            const float Rk = (float)(SL[k].R + 2.0 * kCullMargin);
+            V term;
+            if (k == km) {
+                term = K(Rk);
+            } else {
+                V d[3] = {GP.sub(FV[k][12], c[0]), GP.sub(FV[k][13], c[1]), GP.sub(FV[k][14], c[2])};
+                V dd = GP.dot3(d[0], d[0], d[1], d[1], d[2], d[2]);
+                V sq = dd.k ? K(std::sqrt(dd.c)) : GP.tmp("__builtin_sqrtf(" + GP.ref(dd) + ")");
+                term = GP.add(sq, K(Rk));
+            }
+            if (first) {
+                r = term;
+                first = false;
+            } else if (r.k && term.k) {
+                r = K(std::max(r.c, term.c));
+            } else {
+                r = GP.tmp("fmaxf(" + GP.ref(r) + ", " + GP.ref(term) + ")");
+            }
+        }
+        gsph.push_back({c[0], c[1], c[2], r});
+    }
+    const std::string group_prologue = GP.os.str();
+
     // ---------------------------------------------------------------- environment blocks
     // kind: 0 box, 1 sphere, 2 capsule, 3 cylinder.  The obstacle parameters are named
     // m0..m11 lo{x,y,z} hi{x,y,z} (box), ox oy oz orad (sphere), x1 y1 z1 xv yv zv rdv crad
@@ -439,7 +488,21 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
         Emitter X;
         V mm[12];
         for (int i = 0; i < 12; i++) mm[i] = V{false, 0.0f, "m" + std::to_string(i)};
-        for (int k = 0; k < NL; k++) {
+        // obstacle bounding sphere (centre, radius): the sphere itself, or the record's bound
+        const V oc[3] = {V{false, 0.0f, kind == 1 ? "ox" : "ocx"}, V{false, 0.0f, kind == 1 ? "oy" : "ocy"},
+                         V{false, 0.0f, kind == 1 ? "oz" : "ocz"}};
+        const std::string orr = kind == 1 ? "orad" : "orr";
+        for (size_t gi = 0; gi < groups.size(); gi++) {
+        const bool cull_group = groups.size() > 1;
+        if (cull_group) {
+            X.os << "            {\n";
+            V gd[3] = {X.sub(gsph[gi][0], oc[0]), X.sub(gsph[gi][1], oc[1]), X.sub(gsph[gi][2], oc[2])};
+            V gd2 = X.dot3(gd[0], gd[0], gd[1], gd[1], gd[2], gd[2]);
+            V gs = X.tmp(X.ref(gsph[gi][3]) + " + " + orr);
+            X.os << "            work += " << kSph << "u;\n";
+            X.os << "            if (__ballot(" << X.ref(gd2) << " < " << X.ref(gs) << " * " << X.ref(gs) << ")) {\n";
+        }
+        for (int k : groups[gi]) {
             const SphereLink& L = SL[k];
             float Rm = (float)(L.R + kCullMargin);
             int n = (int)L.spheres.size();
@@ -538,6 +601,8 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
             }
             X.os << "            }\n            }\n";
         }
+        if (cull_group) X.os << "            }\n            }\n";
+        }
         return X.os.str();
     };
 
@@ -562,7 +627,8 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
         "            const float m4 = mp_sld(b + 4), m5 = mp_sld(b + 5), m6 = mp_sld(b + 6), m7 = mp_sld(b + 7);\n"
         "            const float m8 = mp_sld(b + 8), m9 = mp_sld(b + 9), m10 = mp_sld(b + 10), m11 = mp_sld(b + 11);\n"
         "            const float lox = mp_sld(b + 12), loy = mp_sld(b + 13), loz = mp_sld(b + 14);\n"
-        "            const float hix = mp_sld(b + 15), hiy = mp_sld(b + 16), hiz = mp_sld(b + 17);\n";
+        "            const float hix = mp_sld(b + 15), hiy = mp_sld(b + 16), hiz = mp_sld(b + 17);\n"
+        "            const float ocx = mp_sld(b + 18), ocy = mp_sld(b + 19), ocz = mp_sld(b + 20), orr = mp_sld(b + 21);\n";
     const char* sph_load =
         "            const float* p = sc.sph + o * MP_SPH_W;\n"
         "            const float ox = mp_sld(p + 0), oy = mp_sld(p + 1), oz = mp_sld(p + 2), orad = mp_sld(p + 3);\n";
@@ -570,17 +636,19 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
         "            const float* k = sc.cap + o * MP_CAP_W;\n"
         "            const float x1 = mp_sld(k + 0), y1 = mp_sld(k + 1), z1 = mp_sld(k + 2);\n"
         "            const float xv = mp_sld(k + 3), yv = mp_sld(k + 4), zv = mp_sld(k + 5);\n"
-        "            const float rdv = mp_sld(k + 6), crad = mp_sld(k + 7);\n";
+        "            const float rdv = mp_sld(k + 6), crad = mp_sld(k + 7);\n"
+        "            const float ocx = mp_sld(k + 8), ocy = mp_sld(k + 9), ocz = mp_sld(k + 10), orr = mp_sld(k + 11);\n";
     const char* cyl_load =
         "            const float* b = sc.cyl + o * MP_CYL_W;\n"
         "            const float m0 = mp_sld(b + 0), m1 = mp_sld(b + 1), m2 = mp_sld(b + 2), m3 = mp_sld(b + 3);\n"
         "            const float m4 = mp_sld(b + 4), m5 = mp_sld(b + 5), m6 = mp_sld(b + 6), m7 = mp_sld(b + 7);\n"
         "            const float m8 = mp_sld(b + 8), m9 = mp_sld(b + 9), m10 = mp_sld(b + 10), m11 = mp_sld(b + 11);\n"
-        "            const float crad = mp_sld(b + 12), hh = mp_sld(b + 13);\n";
+        "            const float crad = mp_sld(b + 12), hh = mp_sld(b + 13);\n"
+        "            const float ocx = mp_sld(b + 14), ocy = mp_sld(b + 15), ocz = mp_sld(b + 16), orr = mp_sld(b + 17);\n";
     // early-out order: boxes, spheres, capsules, cylinders (cheapest per pair first)
     // in a team, wave w takes the obstacles o with o % W == w of every kind
     o << "    template <int MODE>\n    static __device__ __forceinline__ bool env_hit(const float* F, const SceneDev& sc, bool hit, u64 mask, unsigned& work, const Team& T) {\n";
-    o << "        const int o0 = mp_uniform(T.w), os = mp_uniform(T.W);\n";
+    o << "        const int o0 = mp_uniform(T.w), os = mp_uniform(T.W);\n" << group_prologue;
     o << "        for (int o = o0; o < mp_uniform(sc.nb); o += os) {\n" << box_load << env_block(0, false)
       << "            if (mp_stop<MODE>(hit, mask) || T.stopped()) return hit;\n        }\n";
     o << "        for (int o = o0; o < mp_uniform(sc.ns); o += os) {\n" << sph_load << env_block(1, false)
@@ -593,7 +661,7 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
     // full-evaluation costs in the oracle order: spheres, boxes, cylinders, capsules; culled
     // blocks contribute exact zeros, so skipping them leaves every partial sum unchanged
     o << "    static __device__ __forceinline__ float env_cost(const float* F, const SceneDev& sc, unsigned& work) {\n";
-    o << "        float tot = 0.0f, part = 0.0f;\n";
+    o << "        float tot = 0.0f, part = 0.0f;\n" << group_prologue;
     o << "        for (int o = 0; o < mp_uniform(sc.ns); ++o) {\n" << sph_load << env_block(1, true) << "        }\n";
     o << "        tot += part; part = 0.0f;\n";
     o << "        for (int o = 0; o < mp_uniform(sc.nb); ++o) {\n" << box_load << env_block(0, true) << "        }\n";

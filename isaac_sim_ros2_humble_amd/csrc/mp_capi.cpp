@@ -379,20 +379,90 @@ void inverse_pose(const float* p, const float* R, float* m) {
     m[11] = -dot3f(t20, p[0], t21, p[1], t22, p[2]);
 }
 
+// World bounding sphere of the region {p : m p + t in [lo, hi]} (m, t = the inverse pose a kernel
+// applies, o[0..11]).  Rotations that are not orthonormal to 1e-4 get an infinite radius (never
+// culled): the link-level tests measure distances in the obstacle frame, which equal world
+// distances only for rotations.
+void local_box_bound(const float* o, const float* lo, const float* hi, double wc[3], double& rho) {
+    double M[3][3], t[3];
+    for (int a = 0; a < 3; a++) {
+        for (int c = 0; c < 3; c++) M[a][c] = o[4 * a + c];
+        t[a] = o[4 * a + 3];
+    }
+    double dev = 0.0;
+    for (int a = 0; a < 3; a++)
+        for (int c = 0; c < 3; c++) {
+            double d = M[a][0] * M[c][0] + M[a][1] * M[c][1] + M[a][2] * M[c][2] - (a == c ? 1.0 : 0.0);
+            dev = std::max(dev, std::fabs(d));
+        }
+    // inverse of M by cofactors
+    const double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                       M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+    if (dev > 1e-4 || !(std::fabs(det) > 0.5)) {
+        wc[0] = wc[1] = wc[2] = 0.0;
+        rho = 1e30;
+        return;
+    }
+    double I[3][3];
+    I[0][0] = (M[1][1] * M[2][2] - M[1][2] * M[2][1]) / det;
+    I[0][1] = (M[0][2] * M[2][1] - M[0][1] * M[2][2]) / det;
+    I[0][2] = (M[0][1] * M[1][2] - M[0][2] * M[1][1]) / det;
+    I[1][0] = (M[1][2] * M[2][0] - M[1][0] * M[2][2]) / det;
+    I[1][1] = (M[0][0] * M[2][2] - M[0][2] * M[2][0]) / det;
+    I[1][2] = (M[0][2] * M[1][0] - M[0][0] * M[1][2]) / det;
+    I[2][0] = (M[1][0] * M[2][1] - M[1][1] * M[2][0]) / det;
+    I[2][1] = (M[0][1] * M[2][0] - M[0][0] * M[2][1]) / det;
+    I[2][2] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) / det;
+    auto to_world = [&](const double l[3], double w[3]) {
+        for (int a = 0; a < 3; a++)
+            w[a] = I[a][0] * (l[0] - t[0]) + I[a][1] * (l[1] - t[1]) + I[a][2] * (l[2] - t[2]);
+    };
+    const double lc[3] = {0.5 * ((double)lo[0] + hi[0]), 0.5 * ((double)lo[1] + hi[1]), 0.5 * ((double)lo[2] + hi[2])};
+    to_world(lc, wc);
+    rho = 0.0;
+    for (int corner = 0; corner < 8; corner++) {
+        const double l[3] = {(corner & 1) ? hi[0] : lo[0], (corner & 2) ? hi[1] : lo[1], (corner & 4) ? hi[2] : lo[2]};
+        double w[3];
+        to_world(l, w);
+        rho = std::max(rho, std::sqrt((w[0] - wc[0]) * (w[0] - wc[0]) + (w[1] - wc[1]) * (w[1] - wc[1]) +
+                                      (w[2] - wc[2]) * (w[2] - wc[2])));
+    }
+}
+
+// Bounding sphere of an obstacle for the obstacle-level cull: world centre, radius computed in
+// double and rounded up so that the float sphere contains the obstacle.
+void bound_sphere(float* o, double cx, double cy, double cz, double r) {
+    o[0] = (float)cx;
+    o[1] = (float)cy;
+    o[2] = (float)cz;
+    o[3] = (float)(r * (1.0 + 1e-6) + 1e-6);
+}
+
 void scene_upload(mp_scene* sc, const float* spheres, int ns, const float* boxes, int nb, const float* cyl, int nc,
                   const float* caps, int ncap) {
     std::vector<float> h(sc->offs[3] + (size_t)MP_CAP_W_H * std::max(sc->cap_k, 1), 0.0f);
     for (int i = 0; i < ns; i++) std::memcpy(&h[sc->offs[0] + 4 * i], spheres + 4 * i, 4 * sizeof(float));
     for (int i = 0; i < nb; i++) {
         float* o = &h[sc->offs[1] + MP_BOX_W_H * i];
-        inverse_pose(boxes + 18 * i, boxes + 18 * i + 3, o);
-        std::memcpy(o + 12, boxes + 18 * i + 12, 6 * sizeof(float));
+        const float* b = boxes + 18 * i;
+        inverse_pose(b, b + 3, o);
+        std::memcpy(o + 12, b + 12, 6 * sizeof(float));
+        // the kernel tests local = M p + t (o[0..11]); the box is the preimage of [min, max]:
+        // map its corners back exactly (double) and bound them
+        double wc[3], rho;
+        local_box_bound(o, b + 12, b + 15, wc, rho);
+        bound_sphere(o + 18, wc[0], wc[1], wc[2], rho);
     }
     for (int i = 0; i < nc; i++) {
         float* o = &h[sc->offs[2] + MP_CYL_W_H * i];
         inverse_pose(cyl + 14 * i, cyl + 14 * i + 3, o);
         o[12] = cyl[14 * i + 12];
         o[13] = cyl[14 * i + 13] * 0.5f;
+        // a finite cylinder lies in its local box [-r, r] x [-r, r] x [-h/2, h/2]
+        const float lo[3] = {-o[12], -o[12], -o[13]}, hi[3] = {o[12], o[12], o[13]};
+        double wc[3], rho;
+        local_box_bound(o, lo, hi, wc, rho);
+        bound_sphere(o + 14, wc[0], wc[1], wc[2], rho);
     }
     for (int i = 0; i < ncap; i++) {
         const float* c = caps + 7 * i;
@@ -401,6 +471,9 @@ void scene_upload(mp_scene* sc, const float* spheres, int ns, const float* boxes
         o[3] = c[3] - c[0]; o[4] = c[4] - c[1]; o[5] = c[5] - c[2];
         o[6] = 1.0f / dot3f(o[3], o[3], o[4], o[4], o[5], o[5]);
         o[7] = c[6];
+        const double vx = (double)c[3] - c[0], vy = (double)c[4] - c[1], vz = (double)c[5] - c[2];
+        bound_sphere(o + 8, 0.5 * ((double)c[0] + c[3]), 0.5 * ((double)c[1] + c[4]), 0.5 * ((double)c[2] + c[5]),
+                     0.5 * std::sqrt(vx * vx + vy * vy + vz * vz) + c[6]);
     }
     MP_HIP(hipMemcpyAsync(sc->dev, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, G.stream));
     MP_HIP(hipStreamSynchronize(G.stream));

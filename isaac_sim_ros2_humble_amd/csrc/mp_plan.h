@@ -6,9 +6,9 @@
 #include "mp_internal.h"
 
 #define MP_SPH_W_H 4
-#define MP_BOX_W_H 20
-#define MP_CYL_W_H 16
-#define MP_CAP_W_H 8
+#define MP_BOX_W_H 24
+#define MP_CYL_W_H 20
+#define MP_CAP_W_H 12
 
 namespace mp {
 
