@@ -745,6 +745,7 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "    const int lane = threadIdx.x & (MP_WAVE - 1);\n"
          "    const int n_active = mp_uniform(ctl.ctl[2]);\n"
          "    const int budget = mp_uniform(ctl.ctl[3]);\n"
+         "    const int per_ticket = mp_uniform(ctl.ctl[7]) > 0 ? mp_uniform(ctl.ctl[7]) : 1;\n"
          "    if (n_active <= 0) return;\n"
       << team_setup <<
          "    const bool lead = T.w == 0;\n"
@@ -762,12 +763,11 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "        QueryDev Q;\n"
          "        mp_load_query(ctl.q + qi, Q);\n"
          "        for (int it = 0; budget <= 0 || it < budget; ++it) {\n"
-         "            const int status = mp_team_bcast(lead ? __hip_atomic_load(&st->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0, T);\n"
-         "            if (status != MP_PLAN_RUNNING) break;\n"
+         "            bool more = false;\n"
          "            switch (Q.robot) {\n";
     for (size_t k = 0; k < robots.size(); k++)
-        o << "                case " << k << ": mp_plan_iteration<Robot" << k << ">(ctl, Q, st, pl, T); break;\n";
-    o << "                default: break;\n            }\n        }\n"
+        o << "                case " << k << ": more = mp_plan_iteration<Robot" << k << ">(ctl, Q, st, pl, T); break;\n";
+    o << "                default: break;\n            }\n            if (!more) break;\n        }\n"
          "        mp_prof_add(pl, MP_PROF_WAVE, mp_now(pl) - t_wave);\n"
          "        mp_prof_flush(ctl, pl);\n"
          "        return;\n    }\n"
@@ -807,10 +807,12 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "        QueryDev Q;\n"
          "        mp_load_query(ctl.q + qi, Q);\n"
          "        mp_prof_since(pl, MP_PROF_SCHED, ts);\n"
-         "        switch (Q.robot) {\n";
+         "        for (int rep = 0; rep < per_ticket; ++rep) {\n"
+         "            bool more = false;\n"
+         "            switch (Q.robot) {\n";
     for (size_t k = 0; k < robots.size(); k++)
-        o << "            case " << k << ": mp_plan_iteration<Robot" << k << ">(ctl, Q, st, pl, T); break;\n";
-    o << "            default: break;\n        }\n    }\n"
+        o << "                case " << k << ": more = mp_plan_iteration<Robot" << k << ">(ctl, Q, st, pl, T); break;\n";
+    o << "                default: break;\n            }\n            if (!more) break;\n        }\n    }\n"
          "    mp_prof_add(pl, MP_PROF_WAVE, mp_now(pl) - t_wave);\n"
          "    mp_prof_flush(ctl, pl);\n"
          "}\n";

@@ -47,7 +47,8 @@ struct QueryStateHost {
 static_assert(sizeof(QueryStateHost) == 64, "QueryState layout");
 
 // control block in device memory: [0] ticket, [1] remaining, [2] n_active, [3] budget,
-// [4] first new query (index into the active list), [5] number of new queries
+// [4] first new query (index into the active list), [5] number of new queries, [6] exclusive,
+// [7] iterations per ticket
 struct PlanCtlHost {
     QueryDevHost* q;
     QueryStateHost* st;
@@ -110,6 +111,8 @@ void store_shape(const std::string& set_key, int cls, Shape sh);
 // Fills `out` from a finished query's state; call after the stream that ran the query has synchronised.
 void harvest(const SlotPool& pool, int slot, int dof, const QueryStateHost& st, mp_path* out);
 constexpr int kDefaultTeamWaves = 4;
+// Iterations a team runs on the query its ticket picked (solve paths; the server uses 1).
+constexpr int kItersPerTicket = 4;
 // Resident teams of `team_waves` waves the planning kernel can keep on the device.
 int default_teams(hipFunction_t plan_fn, int team_waves);
 int team_size(const mp_rrtc_params& p);  // waves per team (the default when p.team_waves is 0)

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -175,6 +176,12 @@ void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params
     h[4] = 0;       // first new
     h[5] = nq;      // new count
     h[6] = p.exclusive ? 1 : 0;
+    static const int per_ticket = [] {
+        const char* e = std::getenv("MP_ITERS_PER_TICKET");  // experiment override
+        const int v = e ? std::atoi(e) : 0;
+        return v > 0 ? v : kItersPerTicket;
+    }();
+    h[7] = per_ticket;
     for (int k = 0; k < nq; k++) h[16 + k] = k;
     MP_HIP(hipMemcpyAsync(pool.dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, s));
     MP_HIP(hipMemcpyAsync(pool.dactive, h + 16, (size_t)nq * sizeof(int), hipMemcpyHostToDevice, s));

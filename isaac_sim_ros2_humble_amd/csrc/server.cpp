@@ -91,6 +91,7 @@ void mp_server::step(const std::vector<int>& list, int n_new) {
     h[4] = (int)list.size() - n_new;
     h[5] = n_new;
     h[6] = prm.params.exclusive ? 1 : 0;
+    h[7] = 1;  // one iteration per ticket: the step's budget stays a budget of iterations
     for (size_t k = 0; k < list.size(); k++) h[16 + k] = list[k];
     for (int k = (int)list.size() - n_new; k < (int)list.size(); k++) {
         int slot = list[k];
