@@ -282,6 +282,7 @@ def main():
     ap.add_argument("--tune-queries", type=int, default=128)
     ap.add_argument("--team-waves", type=int, default=0, help="planning launch shape (0: tuned / default)")
     ap.add_argument("--waves", type=int, default=0, help="planning grid in waves (0: tuned / resident)")
+    ap.add_argument("--teams-per-query", type=int, default=0, help="teams per query (0: tuned / unbounded)")
     ap.add_argument("--skip", nargs="*", default=[], help="parts to skip: tune plain serve c2 c3 mbm cpu (cpu also skips c1)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -312,36 +313,49 @@ def main():
         for name, per_query, n in (("batch", False, len(cal)), ("plain", True, min(32, len(cal)))):
             best, cands = native.rrtc_autotune(c_rob[:n], c_sc[:n], c_s[:n], c_g[:n], native.default_params(seed=77),
                                                per_query=per_query, reps=2)
-            tune[name] = {"team_waves": best.team_waves, "waves": best.waves, "calibration_queries": n,
-                          "candidates": cands}
-            log("autotune %s: team_waves %d waves %d" % (name, best.team_waves, best.waves))
-    params = native.default_params(seed=0, team_waves=a.team_waves, waves=a.waves)
+            tune[name] = {"team_waves": best.team_waves, "waves": best.waves, "teams_per_query": best.teams_per_query,
+                          "calibration_queries": n, "candidates": cands}
+            log("autotune %s: team_waves %d waves %d teams_per_query %d" % (name, best.team_waves, best.waves,
+                                                                             best.teams_per_query))
+    params = native.default_params(seed=0, team_waves=a.team_waves, waves=a.waves, teams_per_query=a.teams_per_query)
+
+    # the 256 queries are packed once (host arrays, handles of resident scenes and robots); each
+    # step submits them, runs the persistent launch and has every path on the host
+    req = native.BatchRequest(rob_list, scenes, starts, goals, max_path=params.max_path)
+
+    def step(p):
+        native.sync()
+        t0 = time.perf_counter()
+        r = req.solve(p)
+        native.sync()
+        return r, (time.perf_counter() - t0) * 1e3, native.kernel_ms("plan")
 
     # batched: warmup, then exactly K timed steps
     for _ in range(a.warmup):
-        run_batch(rob_list, scenes, starts, goals, params)
+        step(params)
     step_ms, kern_ms, solved, flops, checked, host_ms = [], [], [], [], [], []
     for k in range(a.steps):
         params.seed = 1000 * (k + 1)
-        res, ms, kms = run_batch(rob_list, scenes, starts, goals, params)
+        res, ms, kms = step(params)
         step_ms.append(ms)
         kern_ms.append(kms)
-        solved.append(sum(r["status"] == "solved" for r in res))
-        flops.append(sum(r["work_flops"] for r in res))
-        checked.append(sum(r["configs_checked"] for r in res))
-        host_ms.append(res[0]["latency_ms"])  # the library's own submit-to-results time
-        log("batch step %d: %.1f ms (kernel %.1f ms), solved %d/%d" % (k, ms, kms, solved[-1], len(res)))
+        solved.append(res.solved)
+        flops.append(float(res.work_flops.sum()))
+        checked.append(float(res.configs_checked.sum()))
+        host_ms.append(float(res.latency_ms[0]))  # the library's own submit-to-results time
+        log("batch step %d: %.1f ms (kernel %.1f ms), solved %d/%d" % (k, ms, kms, solved[-1], req.nq))
     ms_per_step = float(np.mean(step_ms))
     batched_qps = float(np.sum(solved)) / (np.sum(step_ms) / 1e3)
     plan_kernel_ms = float(np.mean(kern_ms))
     achieved = float(np.mean(flops)) / (plan_kernel_ms / 1e3) / 1e12
-    iters = np.array([r["iterations"] for r in res])
-    dev_ms = np.array([r["device_ms"] for r in res])
+    iters = res.iterations
+    dev_ms = res.device_ms
     # one more (untimed) step with the planner's phase profile on: where the waves spend their time
     native.plan_profile(enable=True)
     params.seed = 999
-    shape = {"team_waves": a.team_waves, "waves": a.waves} if a.team_waves or a.waves else tune.get("batch")
-    run_batch(rob_list, scenes, starts, goals, params)
+    shape = ({"team_waves": a.team_waves, "waves": a.waves, "teams_per_query": a.teams_per_query}
+             if a.team_waves or a.waves or a.teams_per_query else tune.get("batch"))
+    step(params)
     prof = native.plan_profile(enable=False)
     wave = max(prof["wave"], 1)
     phase = {k: prof[k] / wave for k in native.PROFILE_PHASES[:8]}
@@ -364,17 +378,19 @@ def main():
            "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic("mp_plan"),
                         "kernel": "mp_plan (batched)", "flops_per_launch": float(np.mean(flops)),
-                        "launch_shape": {"team_waves": shape["team_waves"], "waves": shape["waves"]} if shape else None},
+                        "launch_shape": {k: shape[k] for k in ("team_waves", "waves", "teams_per_query")} if shape else None},
            "autotune": tune}
 
     if "plain" not in a.skip:
+        # the same packed queries, one launch each (query k with seed k), with the plain shape
+        pp = native.default_params(seed=0)
+        native.sync()
         t0 = time.perf_counter()
-        nsol, lat = 0, []
-        for k in range(len(queries)):
-            r = native.rrtc_solve(rob_list[k], scenes[k], starts[k], goals[k], native.default_params(seed=k))
-            nsol += r["status"] == "solved"
-            lat.append(r["latency_ms"])
+        pr = req.solve_each(pp)
+        native.sync()
         dt = time.perf_counter() - t0
+        nsol = pr.solved
+        lat = pr.latency_ms
         plain_qps = nsol / dt
         out["plain"] = {"qps": plain_qps, "solved": nsol, "p50_ms": float(np.percentile(lat, 50)),
                         "p99_ms": float(np.percentile(lat, 99)), "seconds": dt}

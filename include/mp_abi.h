@@ -189,10 +189,13 @@ typedef struct {
     int team_waves;        /* waves per team (1, 2 or 4; 0 = tuned default): the waves of a
                               team share one iteration's nearest-neighbour scans and collision
                               checks (threads per tree iteration = 64 * team_waves) */
+    int teams_per_query;   /* most teams growing one query's trees at once (0 = tuned, or
+                              unbounded): more teams on one tree mostly add redundant
+                              extensions; surplus teams leave the launch */
 } mp_rrtc_params;
 
 /* Defaults: resolution 0.02, max_step 32, max_iterations 10000, tree_capacity 65536,
- * waves 0, max_path 4096, seed 0, exclusive 0, team_waves 0. */
+ * waves 0, max_path 4096, seed 0, exclusive 0, team_waves 0, teams_per_query 0. */
 void mp_rrtc_default_params(mp_rrtc_params* p);
 
 #define MP_PLAN_RUNNING 0
@@ -228,15 +231,17 @@ int mp_rrtc_solve_batch(int nq, mp_robot* const* robots, const mp_scene* const* 
 
 /* Autotuner of the planning launch shape (no counterpart in the reference; CPRRTC fixes one
  * block of threads per worker).  Solves the sample queries once per candidate shape and seed
- * (team_waves 1, 2, 4 x total waves; per_query = 1 tunes the one-query-per-launch shape by
- * solving the sample one query at a time, 0 tunes the batch shape with the sample as one batch),
- * scores each by its mean planning-kernel time, and returns the fastest in *best (params with
- * team_waves and waves set).  The choice is kept for this robot set and class, in memory and in
- * the code-object cache directory, and used by every later solve or server whose params leave
- * team_waves and waves at 0.  results (may be NULL) receives up to cap candidates. */
+ * (team_waves 1, 2, 4 x total waves x teams per query 0, 16, 32, 64, 128; per_query = 1 tunes the
+ * one-query-per-launch shape by solving the sample one query at a time, 0 tunes the batch shape
+ * with the sample as one batch), scores each by its mean planning-kernel time, and returns the
+ * fastest in *best (params with team_waves, waves and teams_per_query set).  The choice is kept
+ * for this robot set and class, in memory and in the code-object cache directory, and used by
+ * every later solve or server whose params leave team_waves, waves and teams_per_query at 0.
+ * results (may be NULL) receives up to cap candidates. */
 typedef struct {
     int team_waves;
     int waves;             /* 0 = every team the device keeps resident */
+    int teams_per_query;   /* 0 = unbounded */
     double kernel_ms;      /* mean planning-kernel time over the seeds (summed over queries
                               when per_query) */
 } mp_tune_result;

@@ -746,6 +746,7 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "    const int n_active = mp_uniform(ctl.ctl[2]);\n"
          "    const int budget = mp_uniform(ctl.ctl[3]);\n"
          "    const int per_ticket = mp_uniform(ctl.ctl[7]) > 0 ? mp_uniform(ctl.ctl[7]) : 1;\n"
+         "    const int cap = mp_uniform(ctl.ctl[8]);\n"
          "    if (n_active <= 0) return;\n"
       << team_setup <<
          "    const bool lead = T.w == 0;\n"
@@ -783,8 +784,9 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "            const int running = mp_uniform(__hip_atomic_load(ctl.ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));\n"
          "            if (!(budget > 0 && t >= (u32)budget) && running > 0) {\n"
          "                qi = -1;\n"
+         "                bool full = false;  // a running query was passed over: it had its teams\n"
          "                const int slot = (int)(t % (u32)n_active);\n"
-         "                for (int base = 0; base < n_active; base += MP_WAVE) {\n"
+         "                for (int base = 0; base < n_active && qi < 0; base += MP_WAVE) {\n"
          "                    const int p = base + lane;\n"
          "                    int a = -1;\n"
          "                    bool run = false;\n"
@@ -794,10 +796,22 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "                        a = ctl.active[k];\n"
          "                        run = __hip_atomic_load(&ctl.st[a].status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == MP_PLAN_RUNNING;\n"
          "                    }\n"
-         "                    const u64 b = __ballot(run);\n"
-         "                    if (b) { qi = __shfl(a, __builtin_ctzll(b)); break; }\n"
+         "                    u64 b = __ballot(run);\n"
+         "                    while (b) {\n"
+         "                        const int cand = __shfl(a, __builtin_ctzll(b));\n"
+         "                        if (cap <= 0) { qi = cand; break; }\n"
+         "                        int old = 0;\n"
+         "                        if (lane == 0) old = __hip_atomic_fetch_add(&ctl.st[cand].workers, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "                        old = __shfl(old, 0);\n"
+         "                        if (old < cap) { qi = cand; break; }\n"
+         "                        if (lane == 0) __hip_atomic_fetch_add(&ctl.st[cand].workers, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "                        full = true;\n"
+         "                        b &= b - 1ull;\n"
+         "                    }\n"
          "                }\n"
          "                qi = mp_uniform(qi);\n"
+         "                // every running query it saw has its teams: this team is surplus and leaves\n"
+         "                if (qi < 0 && full) qi = -2;\n"
          "            }\n"
          "        }\n"
          "        qi = mp_team_bcast(qi, T);\n"
@@ -812,7 +826,9 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "            switch (Q.robot) {\n";
     for (size_t k = 0; k < robots.size(); k++)
         o << "                case " << k << ": more = mp_plan_iteration<Robot" << k << ">(ctl, Q, st, pl, T); break;\n";
-    o << "                default: break;\n            }\n            if (!more) break;\n        }\n    }\n"
+    o << "                default: break;\n            }\n            if (!more) break;\n        }\n"
+         "        if (cap > 0 && lead && lane == 0) __hip_atomic_fetch_add(&st->workers, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "    }\n"
          "    mp_prof_add(pl, MP_PROF_WAVE, mp_now(pl) - t_wave);\n"
          "    mp_prof_flush(ctl, pl);\n"
          "}\n";

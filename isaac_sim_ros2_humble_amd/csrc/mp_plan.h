@@ -39,7 +39,7 @@ struct QueryStateHost {
     int status;
     int conn[2];
     int path_len;
-    int iters;
+    int workers;
     uint64_t checked;
     uint64_t flops;
     uint64_t t_init, t_done;
@@ -98,12 +98,14 @@ size_t slot_bytes(int cap, int dof_max);  // device bytes of one slot
 // Runs f, converting exceptions to an ABI status code (message in mp_last_error()).
 int guard_call(const std::function<void()>& f);
 void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params& p, int team, int teams,
-                       hipStream_t s);
+                       int cap, hipStream_t s);
 
-// Launch shape of the planning kernel: waves per team and total waves (0: resident teams).
+// Launch shape of the planning kernel: waves per team, total waves (0: resident teams) and
+// teams per query (0: unbounded).
 struct Shape {
     int team = 0;
     int waves = 0;
+    int cap = 0;
 };
 std::string robot_set_key(const std::vector<mp_robot*>& robots);
 Shape tuned_shape(const std::string& set_key, int cls);  // team 0 when nothing is tuned

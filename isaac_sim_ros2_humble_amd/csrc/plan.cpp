@@ -149,6 +149,7 @@ void check_params(const mp_rrtc_params* p) {
     if (p->waves < 0) throw Error(MP_E_INVALID, "waves must be >= 0");
     if (p->team_waves != 0 && p->team_waves != 1 && p->team_waves != 2 && p->team_waves != 4)
         throw Error(MP_E_INVALID, "team_waves must be 0, 1, 2 or 4");
+    if (p->teams_per_query < 0) throw Error(MP_E_INVALID, "teams_per_query must be >= 0");
 }
 
 double now_ms() {
@@ -165,7 +166,7 @@ void plan_release_workspaces() {
 // Runs the queries already described in pool.hq[0..nq) to completion in one persistent launch
 // of `teams` workgroups of `team` waves.
 void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params& p, int team, int teams,
-                       hipStream_t s) {
+                       int cap, hipStream_t s) {
     const unsigned block = 64u * (unsigned)team;
     MP_HIP(hipMemcpyAsync(pool.dq, pool.hq, (size_t)nq * sizeof(QueryDevHost), hipMemcpyHostToDevice, s));
     int* h = pool.hpinned;
@@ -182,6 +183,7 @@ void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params
         return v > 0 ? v : kItersPerTicket;
     }();
     h[7] = per_ticket;
+    h[8] = cap;
     for (int k = 0; k < nq; k++) h[16 + k] = k;
     MP_HIP(hipMemcpyAsync(pool.dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, s));
     MP_HIP(hipMemcpyAsync(pool.dactive, h + 16, (size_t)nq * sizeof(int), hipMemcpyHostToDevice, s));
@@ -255,9 +257,9 @@ Shape tuned_shape(const std::string& set_key, int cls) {
     if (it != g_tuned.end()) return it->second.first;
     Shape sh;
     if (FILE* f = std::fopen(tune_file(set_key).c_str(), "r")) {
-        int c, t, w;
-        while (std::fscanf(f, "%d %d %d", &c, &t, &w) == 3)
-            if (c == cls && (t == 1 || t == 2 || t == 4) && w >= 0) sh = Shape{t, w};
+        int c, t, w, q;
+        while (std::fscanf(f, "%d %d %d %d", &c, &t, &w, &q) == 4)
+            if (c == cls && (t == 1 || t == 2 || t == 4) && w >= 0 && q >= 0) sh = Shape{t, w, q};
         std::fclose(f);
     }
     g_tuned[k] = {sh, true};
@@ -273,7 +275,7 @@ void store_shape(const std::string& set_key, int cls, Shape sh) {
         auto it = g_tuned.find(set_key + "@" + std::to_string(c));
         if (it != g_tuned.end() && it->second.first.team > 0)
             lines += std::to_string(c) + " " + std::to_string(it->second.first.team) + " " +
-                     std::to_string(it->second.first.waves) + "\n";
+                     std::to_string(it->second.first.waves) + " " + std::to_string(it->second.first.cap) + "\n";
     }
     const std::string path = tune_file(set_key), tmp = path + ".tmp";
     if (FILE* f = std::fopen(tmp.c_str(), "w")) {
@@ -325,16 +327,18 @@ void solve_impl(const char* tag, int nq, mp_robot* const* robots, const mp_scene
     // the built-in defaults
     int W = params->team_waves > 0 ? params->team_waves : kDefaultTeamWaves;
     int waves = params->waves > 0 ? params->waves : default_wave_count;
-    if (params->team_waves == 0 && params->waves == 0) {
+    int cap = params->teams_per_query;
+    if (params->team_waves == 0 && params->waves == 0 && params->teams_per_query == 0) {
         const Shape tuned = tuned_shape(robot_set_key(uniq), nq == 1 ? 0 : 1);
         if (tuned.team > 0) {
             W = tuned.team;
             waves = tuned.waves;
+            cap = tuned.cap;
         }
     }
     int teams = waves > 0 ? std::max(1, waves / W) : default_teams(mod->fn("mp_plan"), W);
     if (params->exclusive) teams = nq;
-    run_to_completion(*mod, pool, nq, *params, W, teams, s);
+    run_to_completion(*mod, pool, nq, *params, W, teams, cap, s);
     for (int k = 0; k < nq; k++) harvest(pool, k, robots[k]->model.dof, pool.hst_pinned[k], &out[k]);
     double lat = now_ms() - t0;
     for (int k = 0; k < nq; k++) out[k].latency_ms = lat;
@@ -355,6 +359,7 @@ void mp_rrtc_default_params(mp_rrtc_params* p) {
     p->seed = 0;
     p->exclusive = 0;
     p->team_waves = 0;
+    p->teams_per_query = 0;
 }
 
 int mp_rrtc_solve(mp_robot* robot, const mp_scene* scene, const float* start, const float* goal,
@@ -385,48 +390,58 @@ int mp_rrtc_autotune(int nq, mp_robot* const* robots, const mp_scene* const* sce
             if (!robots[k] || !scenes[k]) throw Error(MP_E_INVALID, "null robot or scene in the sample");
         const std::string key = mp::robot_set_key(std::vector<mp_robot*>(robots, robots + nq));
         const int cls = per_query ? 0 : 1;
-        // candidates: waves per team x total waves (0 = every team the device keeps resident)
-        const int team_sizes[] = {1, 2, 4};
-        const int plain_waves[] = {256, 512, 1024, 2048};
-        const int batch_waves[] = {0, 1024, 2048, 4096};
+        // candidates: waves per team x total waves (0 = every team the device keeps resident) x
+        // teams per query (0 = unbounded; single queries only use the grid)
+        struct Cand {
+            int team, waves, cap;
+        };
+        std::vector<Cand> cands;
+        for (int t : {1, 2, 4}) {
+            if (per_query) {
+                for (int w : {256, 512, 1024, 2048}) cands.push_back({t, w, 0});
+            } else {
+                for (int w : {0, 1024})
+                    for (int c : {0, 16, 32, 64, 128}) cands.push_back({t, w, c});
+            }
+        }
         std::vector<mp_path> out((size_t)nq);
-        int bt = 0, bw = 0, n = 0;
+        int n = 0;
+        Cand best_c{0, 0, 0};
         double bms = 1e300;
-        for (int t : team_sizes) {
-            for (int i = 0; i < 4; i++) {
-                mp_rrtc_params p = *params;
-                p.team_waves = t;
-                p.waves = per_query ? plain_waves[i] : batch_waves[i];
-                double ms = 0.0;
-                for (int r = 0; r < reps; r++) {
-                    p.seed = params->seed + 7919ull * (uint64_t)r;
-                    for (auto& o : out) std::memset(&o, 0, sizeof(o));
-                    if (per_query) {
-                        for (int k = 0; k < nq; k++) {
-                            mp::solve_impl("plain", 1, robots + k, scenes + k, starts16 + 16 * k, goals16 + 16 * k, &p,
-                                           &out[k], 512);
-                            ms += mp_last_kernel_ms("plan");
-                        }
-                    } else {
-                        mp::solve_impl("batch", nq, robots, scenes, starts16, goals16, &p, out.data(), 0);
+        for (const Cand& cd : cands) {
+            mp_rrtc_params p = *params;
+            p.team_waves = cd.team;
+            p.waves = cd.waves;
+            p.teams_per_query = cd.cap;
+            double ms = 0.0;
+            for (int r = 0; r < reps; r++) {
+                p.seed = params->seed + 7919ull * (uint64_t)r;
+                for (auto& o : out) std::memset(&o, 0, sizeof(o));
+                if (per_query) {
+                    for (int k = 0; k < nq; k++) {
+                        mp::solve_impl("plain", 1, robots + k, scenes + k, starts16 + 16 * k, goals16 + 16 * k, &p,
+                                       &out[k], 512);
                         ms += mp_last_kernel_ms("plan");
                     }
+                } else {
+                    mp::solve_impl("batch", nq, robots, scenes, starts16, goals16, &p, out.data(), 0);
+                    ms += mp_last_kernel_ms("plan");
                 }
-                ms /= reps;
-                if (results && n < cap) results[n] = mp_tune_result{t, p.waves, ms};
-                n++;
-                if (ms < bms) {
-                    bms = ms;
-                    bt = t;
-                    bw = p.waves;
-                }
+            }
+            ms /= reps;
+            if (results && n < cap) results[n] = mp_tune_result{cd.team, cd.waves, cd.cap, ms};
+            n++;
+            if (ms < bms) {
+                bms = ms;
+                best_c = cd;
             }
         }
         if (n_results) *n_results = n;
-        mp::store_shape(key, cls, mp::Shape{bt, bw});
+        mp::store_shape(key, cls, mp::Shape{best_c.team, best_c.waves, best_c.cap});
         *best = *params;
-        best->team_waves = bt;
-        best->waves = bw;
+        best->team_waves = best_c.team;
+        best->waves = best_c.waves;
+        best->teams_per_query = best_c.cap;
     });
 }
 

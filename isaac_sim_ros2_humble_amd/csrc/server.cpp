@@ -56,7 +56,7 @@ struct mp_server {
     std::vector<int> robot_slot;  // robot index -> the module's robot slot
     std::shared_ptr<mp::Module> mod;
     std::unique_ptr<mp::SlotPool> pool;
-    int nslots = 0, teams = 0, team = 1, budget = 0;
+    int nslots = 0, teams = 0, team = 1, cap = 0, budget = 0;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
@@ -92,6 +92,7 @@ void mp_server::step(const std::vector<int>& list, int n_new) {
     h[5] = n_new;
     h[6] = prm.params.exclusive ? 1 : 0;
     h[7] = 1;  // one iteration per ticket: the step's budget stays a budget of iterations
+    h[8] = cap;
     for (size_t k = 0; k < list.size(); k++) h[16 + k] = list[k];
     for (int k = (int)list.size() - n_new; k < (int)list.size(); k++) {
         int slot = list[k];
@@ -234,13 +235,16 @@ int mp_server_create(const mp_server_params* params, mp_server** out) {
         s->pool = std::make_unique<mp::SlotPool>(s->nslots, params->tree_capacity, dof_max, p.max_path);
         if (p.team_waves != 0 && p.team_waves != 1 && p.team_waves != 2 && p.team_waves != 4)
             throw Error(MP_E_INVALID, "team_waves must be 0, 1, 2 or 4");
+        if (p.teams_per_query < 0) throw Error(MP_E_INVALID, "teams_per_query must be >= 0");
         s->team = mp::team_size(p);
+        s->cap = p.teams_per_query;
         int waves = params->waves;
-        if (p.team_waves == 0 && params->waves == 0) {  // the tuned batch shape of this robot set
+        if (p.team_waves == 0 && params->waves == 0 && p.teams_per_query == 0) {  // the tuned batch shape
             const mp::Shape tuned = mp::tuned_shape(mp::robot_set_key(s->robots), 1);
             if (tuned.team > 0) {
                 s->team = tuned.team;
                 waves = tuned.waves;
+                s->cap = tuned.cap;
             }
         }
         s->teams = waves > 0 ? std::max(1, waves / s->team) : mp::default_teams(s->mod->fn("mp_plan"), s->team);

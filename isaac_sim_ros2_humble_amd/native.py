@@ -30,7 +30,7 @@ class RobotDesc(C.Structure):
 class RRTCParams(C.Structure):
     _fields_ = [("resolution", C.c_float), ("max_step", C.c_int), ("max_iterations", C.c_int),
                 ("tree_capacity", C.c_int), ("waves", C.c_int), ("max_path", C.c_int), ("seed", C.c_uint64),
-                ("exclusive", C.c_int), ("team_waves", C.c_int)]
+                ("exclusive", C.c_int), ("team_waves", C.c_int), ("teams_per_query", C.c_int)]
 
 
 class Path(C.Structure):
@@ -41,7 +41,7 @@ class Path(C.Structure):
 
 
 class TuneResult(C.Structure):
-    _fields_ = [("team_waves", C.c_int), ("waves", C.c_int), ("kernel_ms", C.c_double)]
+    _fields_ = [("team_waves", C.c_int), ("waves", C.c_int), ("teams_per_query", C.c_int), ("kernel_ms", C.c_double)]
 
 
 class ServerParams(C.Structure):
@@ -381,45 +381,100 @@ PATH_DTYPE = np.dtype([("status", "<i4"), ("num_waypoints", "<i4"), ("waypoints"
 assert PATH_DTYPE.itemsize == C.sizeof(Path) and all(
     PATH_DTYPE.fields[n][1] == getattr(Path, n).offset for n, _ in Path._fields_)
 
-_WP_BLOCK = [np.empty((0,), np.float32)]  # waypoint storage reused across batches (pages stay mapped)
+class BatchRequest:
+    """A set of planning queries packed once for repeated solves: start/goal vectors (16-float
+    rows), robot and scene handles, an mp_path array and its waypoint storage.
+
+    solve(params): all queries in one persistent launch (mp_rrtc_solve_batch).
+    solve_each(params): the same queries one launch each (mp_rrtc_solve), query k with seed
+    params.seed + k, as in the batch.
+    Both return a BatchResult; a result's paths stay valid until the request is solved again."""
+
+    def __init__(self, robots, scenes, starts, goals, max_path=4096):
+        self.nq = len(robots)
+        self.dof = np.array([r.dof for r in robots], np.int64)
+        self.s16 = np.zeros((self.nq, 16), np.float32)
+        self.g16 = np.zeros((self.nq, 16), np.float32)
+        for k in range(self.nq):
+            self.s16[k, :self.dof[k]] = starts[k]
+            self.g16[k, :self.dof[k]] = goals[k]
+        self.rh = (C.c_void_p * self.nq)(*[r.h for r in robots])
+        self.sh = (C.c_void_p * self.nq)(*[s.h for s in scenes])
+        self.max_path = max_path
+        self.stride = max_path * 16
+        self.block = np.zeros(self.nq * self.stride, np.float32)
+        self.paths = np.zeros(self.nq, PATH_DTYPE)
+        self.paths["waypoints"] = self.block.ctypes.data + np.arange(self.nq, dtype=np.uint64) * np.uint64(self.stride * 4)
+        self.paths["capacity"] = max_path
+        self.generation = 0
+        self._keep = (list(robots), list(scenes))
+
+    def _check_params(self, params):
+        if params.max_path > self.max_path:
+            raise MPError("params.max_path %d exceeds the request's %d" % (params.max_path, self.max_path))
+
+    def solve(self, params=None):
+        params = params or default_params(max_path=self.max_path)
+        self._check_params(params)
+        check(lib().mp_rrtc_solve_batch(self.nq, C.cast(self.rh, C.c_void_p), C.cast(self.sh, C.c_void_p),
+                                        _p(self.s16), _p(self.g16), C.byref(params),
+                                        self.paths.ctypes.data_as(C.c_void_p)), "mp_rrtc_solve_batch")
+        self.generation += 1
+        return BatchResult(self)
+
+    def solve_each(self, params=None):
+        params = params or default_params(max_path=self.max_path)
+        self._check_params(params)
+        p = RRTCParams.from_buffer_copy(params)
+        base = params.seed
+        L = lib()
+        s0, g0, p0 = self.s16.ctypes.data, self.g16.ctypes.data, self.paths.ctypes.data
+        for k in range(self.nq):
+            p.seed = base + k
+            check(L.mp_rrtc_solve(self.rh[k], self.sh[k], s0 + 64 * k, g0 + 64 * k, C.byref(p),
+                                  C.cast(p0 + PATH_DTYPE.itemsize * k, C.POINTER(Path))), "mp_rrtc_solve")
+        self.generation += 1
+        return BatchResult(self)
 
 
-def _waypoint_block(n):
-    if _WP_BLOCK[0].size < n:
-        _WP_BLOCK[0] = np.zeros(n, np.float32)
-    return _WP_BLOCK[0]
+class BatchResult:
+    """Per-query outcome columns of a solved BatchRequest (numpy arrays) and path access."""
+
+    def __init__(self, req):
+        self._req = req
+        self._gen = req.generation
+        r = req.paths.copy()
+        self.code = r["status"]
+        self.status = [PLAN_STATUS.get(int(c), str(int(c))) for c in self.code]
+        self.solved = int((self.code == 1).sum())
+        self.num_waypoints = r["num_waypoints"]
+        self.iterations = r["iterations"]
+        self.start_tree = r["start_tree_size"]
+        self.goal_tree = r["goal_tree_size"]
+        self.configs_checked = r["configs_checked"]
+        self.latency_ms = r["latency_ms"]
+        self.work_flops = r["work_flops"]
+        self.device_ms = r["device_ms"]
+
+    def path(self, k):
+        if self._req.generation != self._gen:
+            raise MPError("the request has been solved again: its paths were overwritten")
+        n = max(int(self.num_waypoints[k]), 0) if self.code[k] == 1 else 0
+        d = int(self._req.dof[k])
+        o = k * self._req.stride
+        return self._req.block[o:o + n * d].reshape(n, d).copy()
+
+    def to_dicts(self):
+        return [{"status": self.status[k], "path": self.path(k), "iterations": int(self.iterations[k]),
+                 "start_tree": int(self.start_tree[k]), "goal_tree": int(self.goal_tree[k]),
+                 "configs_checked": int(self.configs_checked[k]), "latency_ms": float(self.latency_ms[k]),
+                 "work_flops": float(self.work_flops[k]), "device_ms": float(self.device_ms[k])}
+                for k in range(len(self.code))]
 
 
 def rrtc_solve_batch(robots, scenes, starts, goals, params=None):
     params = params or default_params()
-    nq = len(robots)
-    dof = np.array([r.dof for r in robots])
-    s16 = np.zeros((nq, 16), np.float32)
-    g16 = np.zeros((nq, 16), np.float32)
-    for k in range(nq):
-        s16[k, :dof[k]] = starts[k]
-        g16[k, :dof[k]] = goals[k]
-    stride = params.max_path * 16
-    block = _waypoint_block(nq * stride)
-    paths = np.zeros(nq, PATH_DTYPE)
-    paths["waypoints"] = block.ctypes.data + np.arange(nq, dtype=np.uint64) * np.uint64(stride * 4)
-    paths["capacity"] = params.max_path
-    rh = (C.c_void_p * nq)(*[r.h for r in robots])
-    sh = (C.c_void_p * nq)(*[s.h for s in scenes])
-    check(lib().mp_rrtc_solve_batch(nq, C.cast(rh, C.c_void_p), C.cast(sh, C.c_void_p), _p(s16), _p(g16),
-                                    C.byref(params), paths.ctypes.data_as(C.c_void_p)), "mp_rrtc_solve_batch")
-    out = []
-    for k in range(nq):
-        r = paths[k]
-        st = int(r["status"])
-        n = max(int(r["num_waypoints"]), 0) if st == 1 else 0
-        d = int(dof[k])
-        out.append({"status": PLAN_STATUS.get(st, str(st)), "path": block[k * stride:k * stride + n * d].reshape(n, d).copy(),
-                    "iterations": int(r["iterations"]), "start_tree": int(r["start_tree_size"]),
-                    "goal_tree": int(r["goal_tree_size"]), "configs_checked": int(r["configs_checked"]),
-                    "latency_ms": float(r["latency_ms"]), "work_flops": float(r["work_flops"]),
-                    "device_ms": float(r["device_ms"])})
-    return out
+    return BatchRequest(robots, scenes, starts, goals, max_path=params.max_path).solve(params).to_dicts()
 
 
 def rrtc_autotune(robots, scenes, starts, goals, params=None, per_query=False, reps=2):
@@ -440,7 +495,8 @@ def rrtc_autotune(robots, scenes, starts, goals, params=None, per_query=False, r
     check(lib().mp_rrtc_autotune(nq, C.cast(rh, C.c_void_p), C.cast(sh, C.c_void_p), _p(s16), _p(g16), C.byref(params),
                                  1 if per_query else 0, reps, C.cast(res, C.c_void_p), 64, C.byref(n), C.byref(best)),
           "mp_rrtc_autotune")
-    return best, [{"team_waves": r.team_waves, "waves": r.waves, "kernel_ms": r.kernel_ms} for r in res[:min(n.value, 64)]]
+    return best, [{"team_waves": r.team_waves, "waves": r.waves, "teams_per_query": r.teams_per_query,
+                   "kernel_ms": r.kernel_ms} for r in res[:min(n.value, 64)]]
 
 
 class Server:

@@ -188,11 +188,37 @@ def test_autotune_picks_fastest_candidate_and_is_used(fr3, fetch):
             continue
         robots.append(_robot(m)); scenes.append(native.Scene(scene)); starts.append(pair[0]); goals.append(pair[1])
     best, cands = native.rrtc_autotune(robots, scenes, starts, goals, native.default_params(seed=4), reps=1)
-    assert len(cands) == 12
+    assert len(cands) == 30
     assert {c["team_waves"] for c in cands} == {1, 2, 4}
+    assert {c["teams_per_query"] for c in cands} == {0, 16, 32, 64, 128}
     fastest = min(cands, key=lambda c: c["kernel_ms"])
-    assert (best.team_waves, best.waves) == (fastest["team_waves"], fastest["waves"])
+    assert (best.team_waves, best.waves, best.teams_per_query) == (
+        fastest["team_waves"], fastest["waves"], fastest["teams_per_query"])
     assert all(c["kernel_ms"] > 0 for c in cands)
     # later batches with team_waves = waves = 0 run with the stored shape
     res = native.rrtc_solve_batch(robots, scenes, starts, goals, native.default_params(seed=9))
     assert sum(r["status"] == "solved" for r in res) >= 0.8 * len(res)
+
+
+def test_batch_request_batch_and_each_agree_in_exclusive_mode(fr3, fetch):
+    """A packed request solved in one launch and one launch per query (query k with seed + k)
+    gives the same results in exclusive mode; paths are readable until the next solve."""
+    rng = np.random.default_rng(41)
+    robots, scenes, starts, goals = [], [], [], []
+    for k in range(8):
+        m = fr3 if k % 2 == 0 else fetch
+        scene = workloads.random_scene(rng, workloads.log_uniform_count(rng, 5, 80), *workloads.workspace(m))
+        pair = workloads.sample_pair(m, scene, rng, gpu_check, gpu_edge, want_free_line=(k % 4 == 0))
+        assert pair is not None
+        robots.append(_robot(m)); scenes.append(native.Scene(scene)); starts.append(pair[0]); goals.append(pair[1])
+    req = native.BatchRequest(robots, scenes, starts, goals)
+    p = native.default_params(exclusive=1, seed=17)
+    a = req.solve(p)
+    pa = [a.path(k) for k in range(req.nq)]
+    b = req.solve_each(p)
+    with pytest.raises(native.MPError):
+        a.path(0)
+    assert list(a.status) == list(b.status) and a.solved >= 6
+    assert np.array_equal(a.iterations, b.iterations)
+    for k in range(req.nq):
+        assert np.array_equal(pa[k], b.path(k))
