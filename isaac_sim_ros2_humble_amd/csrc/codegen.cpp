@@ -428,7 +428,7 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
                     X.os << "            h |= " << X.ref(dd) << " < " << lit(std::min(thr, sqrt_threshold(sr))) << ";\n";
             }
             X.os << "        }\n        }\n";
-            if (!cost) X.os << "        if (mp_stop<MODE>(h, mask) || T.stopped()) return h;\n        }\n";
+            if (!cost) X.os << "        if (mp_stop<MODE>(h, mask)) return h;\n        }\n";
         }
         return X.os.str();
     };
@@ -492,16 +492,24 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
         const V oc[3] = {V{false, 0.0f, kind == 1 ? "ox" : "ocx"}, V{false, 0.0f, kind == 1 ? "oy" : "ocy"},
                          V{false, 0.0f, kind == 1 ? "oz" : "ocz"}};
         const std::string orr = kind == 1 ? "orad" : "orr";
-        for (size_t gi = 0; gi < groups.size(); gi++) {
         const bool cull_group = groups.size() > 1;
+        // all group tests of this obstacle first (straight-line code, no branch in between), then
+        // one branch for the common case that no lane is near any group
         if (cull_group) {
-            X.os << "            {\n";
-            V gd[3] = {X.sub(gsph[gi][0], oc[0]), X.sub(gsph[gi][1], oc[1]), X.sub(gsph[gi][2], oc[2])};
-            V gd2 = X.dot3(gd[0], gd[0], gd[1], gd[1], gd[2], gd[2]);
-            V gs = X.tmp(X.ref(gsph[gi][3]) + " + " + orr);
-            X.os << "            work += " << kSph << "u;\n";
-            X.os << "            if (__ballot(" << X.ref(gd2) << " < " << X.ref(gs) << " * " << X.ref(gs) << ")) {\n";
+            std::string any;
+            for (size_t gi = 0; gi < groups.size(); gi++) {
+                V gd[3] = {X.sub(gsph[gi][0], oc[0]), X.sub(gsph[gi][1], oc[1]), X.sub(gsph[gi][2], oc[2])};
+                V gd2 = X.dot3(gd[0], gd[0], gd[1], gd[1], gd[2], gd[2]);
+                V gs = X.tmp(X.ref(gsph[gi][3]) + " + " + orr);
+                X.os << "            const u64 gb" << gi << " = __ballot(" << X.ref(gd2) << " < " << X.ref(gs) << " * "
+                     << X.ref(gs) << ");\n";
+                any += (gi ? " | gb" : "gb") + std::to_string(gi);
+            }
+            X.os << "            work += " << kSph * (int)groups.size() << "u;\n";
+            X.os << "            if (" << any << ") {\n";
         }
+        for (size_t gi = 0; gi < groups.size(); gi++) {
+        if (cull_group) X.os << "            {\n            if (gb" << gi << ") {\n";
         for (int k : groups[gi]) {
             const SphereLink& L = SL[k];
             float Rm = (float)(L.R + kCullMargin);
@@ -603,6 +611,7 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
         }
         if (cull_group) X.os << "            }\n            }\n";
         }
+        if (cull_group) X.os << "            }\n";
         return X.os.str();
     };
 
@@ -646,17 +655,18 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
         "            const float crad = mp_sld(b + 12), hh = mp_sld(b + 13);\n"
         "            const float ocx = mp_sld(b + 14), ocy = mp_sld(b + 15), ocz = mp_sld(b + 16), orr = mp_sld(b + 17);\n";
     // early-out order: boxes, spheres, capsules, cylinders (cheapest per pair first)
-    // in a team, wave w takes the obstacles o with o % W == w of every kind
+    // in a team, wave w takes the obstacles o with o % W == w of every kind; the team's early-exit
+    // flag (an LDS read) is looked at every fourth obstacle
     o << "    template <int MODE>\n    static __device__ __forceinline__ bool env_hit(const float* F, const SceneDev& sc, bool hit, u64 mask, unsigned& work, const Team& T) {\n";
     o << "        const int o0 = mp_uniform(T.w), os = mp_uniform(T.W);\n" << group_prologue;
-    o << "        for (int o = o0; o < mp_uniform(sc.nb); o += os) {\n" << box_load << env_block(0, false)
-      << "            if (mp_stop<MODE>(hit, mask) || T.stopped()) return hit;\n        }\n";
-    o << "        for (int o = o0; o < mp_uniform(sc.ns); o += os) {\n" << sph_load << env_block(1, false)
-      << "            if (mp_stop<MODE>(hit, mask) || T.stopped()) return hit;\n        }\n";
-    o << "        for (int o = o0; o < mp_uniform(sc.ncap); o += os) {\n" << cap_load << env_block(2, false)
-      << "            if (mp_stop<MODE>(hit, mask) || T.stopped()) return hit;\n        }\n";
-    o << "        for (int o = o0; o < mp_uniform(sc.nc); o += os) {\n" << cyl_load << env_block(3, false)
-      << "            if (mp_stop<MODE>(hit, mask) || T.stopped()) return hit;\n        }\n";
+    o << "        for (int o = o0, it = 1; o < mp_uniform(sc.nb); o += os, ++it) {\n" << box_load << env_block(0, false)
+      << "            if (mp_stop<MODE>(hit, mask) || ((it & 3) == 0 && T.stopped())) return hit;\n        }\n";
+    o << "        for (int o = o0, it = 1; o < mp_uniform(sc.ns); o += os, ++it) {\n" << sph_load << env_block(1, false)
+      << "            if (mp_stop<MODE>(hit, mask) || ((it & 3) == 0 && T.stopped())) return hit;\n        }\n";
+    o << "        for (int o = o0, it = 1; o < mp_uniform(sc.ncap); o += os, ++it) {\n" << cap_load << env_block(2, false)
+      << "            if (mp_stop<MODE>(hit, mask) || ((it & 3) == 0 && T.stopped())) return hit;\n        }\n";
+    o << "        for (int o = o0, it = 1; o < mp_uniform(sc.nc); o += os, ++it) {\n" << cyl_load << env_block(3, false)
+      << "            if (mp_stop<MODE>(hit, mask) || ((it & 3) == 0 && T.stopped())) return hit;\n        }\n";
     o << "        return hit;\n    }\n";
     // full-evaluation costs in the oracle order: spheres, boxes, cylinders, capsules; culled
     // blocks contribute exact zeros, so skipping them leaves every partial sum unchanged

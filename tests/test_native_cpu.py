@@ -53,7 +53,7 @@ def test_generated_checks_split_across_a_team(fr3):
     assert blocks == len(set(int(x.split(")")[0]) for x in hit.split("T.mine(")[1:]))
     assert [int(x.split(")")[0]) for x in hit.split("T.mine(")[1:]] == list(range(blocks))
     env = r0[r0.index("env_hit"):r0.index("env_cost")]
-    assert env.count("o += os)") == 4 and "const int o0 = mp_uniform(T.w), os = mp_uniform(T.W);" in env
+    assert env.count("o += os, ++it)") == 4 and "const int o0 = mp_uniform(T.w), os = mp_uniform(T.W);" in env
 
 
 def test_generated_source_is_deterministic_and_robot_specific(fr3):
