@@ -224,7 +224,7 @@ def bench_c3(robot, model, seeds):
 def bench_mbm(models):
     """MotionBenchMaker problems (committed fixtures: 5 per set, 14 sets) in one batched launch."""
     from isaac_sim_ros2_humble_amd import mbm
-    mm = [mbm.mbm_model(m) for m in models]
+    mm = mbm.problem_models()
     rob = [native.Robot(m) for m in mm]
     probs = mbm.load_problems(os.path.join(REPO, "tests", "golden", "mbm"), mm)
     scenes = [native.Scene(p[2]) for p in probs]

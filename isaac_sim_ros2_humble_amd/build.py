@@ -77,7 +77,7 @@ def precompile_bundled():
     import multiprocessing as mp
     import glob
     import time
-    sets = [(), ("fr3",), ("fetch",), ("fr3", "fetch"), ("fr3_mbm",), ("fetch_mbm",), ("fr3_mbm", "fetch_mbm")]
+    sets = [(), ("fr3",), ("fetch",), ("fr3", "fetch"), ("fer_mbm",), ("fetch_mbm",), ("fer_mbm", "fetch_mbm")]
     t0 = time.time() - 1.0
     with cf.ProcessPoolExecutor(len(sets), mp_context=mp.get_context("spawn")) as ex:  # noqa: SIM117
         for names in ex.map(_precompile_one, sets):

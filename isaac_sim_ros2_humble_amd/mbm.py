@@ -11,8 +11,11 @@ obstacle_generator.hpp:914-975 for scenes, MPM_helper.hpp:10-92 for requests):
 Inactive joints of the robot take their values from the request's start state, as the
 reference's MBM harness does (cuda_test.cpp:2806-2837 with robot_info.hpp:127-155).
 
-Panda problems name joints panda_jointN / panda_finger_jointN; they map onto the FR3 model's
-fr3_* joints (same kinematic parameters).
+Panda problems name joints panda_jointN / panda_finger_jointN.  They are planned for the FER
+model (robots/fer.json: franka_description/robots/fer, the Panda arm), whose kinematics.yaml
+equals FR3's but whose joint limits are the Panda's; the FER model keeps FR3's fr3_* joint names,
+onto which the panda_* names map.  Under FR3's narrower limits many Panda goals lie out of
+bounds.
 """
 import copy
 
@@ -103,8 +106,11 @@ def with_defaults(model, values):
 MBM_INACTIVE = {
     "fetch": {"bellows_joint": 0.05, "head_pan_joint": 0.0, "head_tilt_joint": 0.0, "l_gripper_finger_joint": 0.05,
               "l_wheel_joint": 0.0, "r_gripper_finger_joint": 0.05, "r_wheel_joint": 0.0},
-    "fr3": {"fr3_finger_joint1": 0.065, "fr3_finger_joint2": 0.065},
+    "fer": {"fr3_finger_joint1": 0.065, "fr3_finger_joint2": 0.065},
 }
+
+# Robot model of each problem family: sets named *_panda use FER, *_fetch use Fetch.
+MBM_ROBOTS = ("fer", "fetch")
 
 
 def mbm_model(model):
@@ -114,8 +120,15 @@ def mbm_model(model):
     return m
 
 
+def problem_models():
+    """[FER, Fetch] as configured by the MBM requests: the models `load_problems` indexes."""
+    from .model import load_robot
+    return [mbm_model(load_robot(n)) for n in MBM_ROBOTS]
+
+
 def load_problems(root, models):
-    """All (set name, model index, Scene, start, goal) under `root`/<set>/scene*.yaml."""
+    """All (set name, model index, Scene, start, goal) under `root`/<set>/scene*.yaml;
+    `models` = [Panda-family model, Fetch model], normally `problem_models()`."""
     import glob
     import os
     out = []

@@ -208,12 +208,16 @@ def parse_fetch(ref):
     return build_model("fetch", group, links, joints, active, sy, disabled)
 
 
-def parse_fr3(ref):
+def parse_fr3(ref, arm="fr3"):
+    """FR3, or with arm="fer" the FER (Panda) arm: franka_description/robots/fer has the same
+    kinematics.yaml as robots/fr3 but its own joint_limits.yaml.  The FER model keeps the FR3
+    link and joint names, sphere model and SRDF pairs (the Panda problems of MotionBenchMaker
+    name joints panda_jointN, mapped onto these by the problem loader); only the bounds differ."""
     base = os.path.join(ref, "src/franka_ros2")
     desc = os.path.join(base, "franka_description")
-    with open(os.path.join(desc, "robots/fr3/kinematics.yaml")) as f:
+    with open(os.path.join(desc, "robots/%s/kinematics.yaml" % arm)) as f:
         kin = yaml.safe_load(f)
-    with open(os.path.join(desc, "robots/fr3/joint_limits.yaml")) as f:
+    with open(os.path.join(desc, "robots/%s/joint_limits.yaml" % arm)) as f:
         lim = yaml.safe_load(f)
     p = "fr3_"
     links = ["base"] + [p + "link%d" % i for i in range(9)] + [p + "hand", p + "hand_tcp", p + "leftfinger", p + "rightfinger"]
@@ -246,7 +250,7 @@ def parse_fr3(ref):
             disabled.append((m.group(1), m.group(2)))
     with open(os.path.join(desc, "robots/franka.yml")) as f:
         sy = yaml.safe_load(f)["collision_spheres"]
-    return build_model("fr3", "fr3_arm", links, joints, active, sy, disabled)
+    return build_model(arm, "fr3_arm", links, joints, active, sy, disabled)
 
 
 def main():
@@ -254,7 +258,8 @@ def main():
     ap.add_argument("--reference", default="/root/reference")
     ap.add_argument("--out", default=HERE)
     a = ap.parse_args()
-    for fn, model in (("fr3.json", parse_fr3(a.reference)), ("fetch.json", parse_fetch(a.reference))):
+    for fn, model in (("fr3.json", parse_fr3(a.reference)), ("fer.json", parse_fr3(a.reference, "fer")),
+                      ("fetch.json", parse_fetch(a.reference))):
         with open(os.path.join(a.out, fn), "w") as f:
             json.dump(model, f, indent=1)
         print(fn, "links", len(model["links"]), "dof", model["dof"], "spheres", len(model["sphere_radius"]),
