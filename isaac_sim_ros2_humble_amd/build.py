@@ -49,7 +49,10 @@ def _precompile_one(names):
     from isaac_sim_ros2_humble_amd import mbm, native
     from isaac_sim_ros2_humble_amd.model import load_robot
     models = [mbm.mbm_model(load_robot(n[:-4])) if n.endswith("_mbm") else load_robot(n) for n in names]
-    native.precompile(models)
+    try:
+        native.precompile(models)
+    except Exception as e:  # the parent may not import this package: hand back a plain error
+        raise RuntimeError("precompiling %s: %s" % (names, e)) from None
     return names
 
 
