@@ -769,7 +769,7 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "        // exclusive: team b alone grows query active[b] (deterministic per seed)\n"
          "        const int b = (int)blockIdx.x;\n"
          "        if (b >= n_active) return;\n"
-         "        const int qi = ctl.active[b];\n"
+         "        const int qi = mp_uniform(ctl.active[b]);\n"
          "        QueryState* st = ctl.st + qi;\n"
          "        QueryDev Q;\n"
          "        mp_load_query(ctl.q + qi, Q);\n"
@@ -824,7 +824,8 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "                if (qi < 0 && full) qi = -2;\n"
          "            }\n"
          "        }\n"
-         "        qi = mp_team_bcast(qi, T);\n"
+         "        // wave-uniform: the descriptor and scene loads below then use scalar loads\n"
+         "        qi = mp_uniform(mp_team_bcast(qi, T));\n"
          "        if (qi == -2) break;\n"
          "        if (qi < 0) { mp_prof_since(pl, MP_PROF_SCHED, ts); continue; }\n"
          "        QueryState* st = ctl.st + qi;\n"
@@ -848,7 +849,7 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "    if (b >= count) return;\n"
       << team_setup <<
          "    __syncthreads();\n"
-         "    const int qi = ctl.active[first + b];\n"
+         "    const int qi = mp_uniform(ctl.active[first + b]);\n"
          "    QueryDev Q;\n"
          "    mp_load_query(ctl.q + qi, Q);\n"
          "    for (int j = 0; j < 16; ++j) { Q.start[j] = ctl.q[qi].start[j]; Q.goal[j] = ctl.q[qi].goal[j]; }\n"
