@@ -172,6 +172,8 @@ class SingleArmSpace : public BaseSpace {
     // access for the planners of this library
     mp_robot* robot() const { return robot_; }
     mp_scene* scene() const { return scene_; }
+    // n samples of the space's stream written to device memory d_q[n][dim] (as sample(n))
+    void sampleInto(float* d_q, int n);
 
   private:
     std::vector<float> lower_bound, upper_bound, defaults_;
@@ -187,6 +189,9 @@ typedef std::shared_ptr<SingleArmSpace> SingleArmSpacePtr;
 // --------------------------------------------------------------------------------- util
 std::vector<std::vector<float>> interpolateVectors(const std::vector<float>& v1, const std::vector<float>& v2,
                                                    float resolution);
+// vectors of vec without those equal (componentwise within tol) to an earlier kept one
+// (util.cpp:250-267)
+std::vector<std::vector<float>> removeRedundantVectors(const std::vector<std::vector<float>>& vec, float tol);
 
 // ------------------------------------------------------------------------------------- tasks
 class BaseTask {
@@ -211,6 +216,11 @@ class SingleArmTask : public BaseTask {
     SingleArmTask(const std::vector<std::vector<float>>& start_joint_values,
                   const std::vector<std::vector<float>>& goal_joint_values)
         : start_(start_joint_values), goal_(goal_joint_values) {}
+    // Goal region (SingleArmTask.cpp:12-37): the free states among goal_config_sampling_attempts
+    // samples of the goal-region space, without repeats (componentwise within 1e-5).  The
+    // region's constraints are those a SingleArmSpace checks; there is no projection step.
+    SingleArmTask(const std::vector<std::vector<float>>& start_joint_values, const BaseSpacePtr& goal_region,
+                  int goal_config_sampling_attempts = 50);
     std::vector<std::vector<float>> getStartStatesVector() override { return start_; }
     std::vector<std::vector<float>> getGoalStatesVector() override { return goal_; }
     void setSolution(const std::vector<std::vector<float>>& solution) override {
@@ -295,18 +305,38 @@ class cRRTC : public BasePlanner {
     BaseTaskPtr task_;
     int max_iterations_ = 10000;
 };
-// RRG (RRG.cpp:141-531) over the same device planner: start/goal validation, direct-connection
-// check, then device RRT-Connect attempts until success or termination.
-class RRG : public cRRTC {
+// RRG (RRG.h:12-130, RRG.cpp:9-544): batched extend-connect roadmap between a start group and a
+// goal group.  Each iteration samples k states, steers them toward their nearest node of one
+// group (the groups alternate), keeps the free ones that reach the nearest node of either group
+// by a free motion, and adds them with those edges; a state that reaches both groups closes a
+// path, which Dijkstra recovers.  A side whose samples reach none of its nodes halves its travel
+// distance (floor 0.1) and goes again.  Sampling, nearest neighbours, steering and the state and
+// motion checks run on the device in batches; the graph (a few thousand vertices) is host-side.
+class RRG : public BasePlanner {
   public:
-    RRG(BaseSpacePtr space) : cRRTC(space) {}
+    RRG(BaseSpacePtr space) : BasePlanner(space) {}
+    void setMotionTask(BaseTaskPtr task, bool get_full_path = true) override;
+    void solve(BaseTerminationPtr termination_condition) override;
     void setMaxTravelDistance(float d) { max_travel_distance_ = d; }
     void setSampleAttemptsInEachIteration(int n) { sample_attempts_ = n; }
-    void solve(BaseTerminationPtr termination_condition) override;
+    // states of the start group and of the goal group (RRG.cpp:531-544)
+    void getStartAndGoalGroupStates(BaseStatesPtr& start_group_states, BaseStatesPtr& goal_group_states);
 
   private:
+    struct Edge {
+        int to;
+        float weight;
+    };
+    BaseTaskPtr task_;
     float max_travel_distance_ = 0.5f;
     int sample_attempts_ = 30;
+    std::vector<std::vector<float>> node_;  // joint values per vertex (empty for the two roots)
+    std::vector<std::vector<Edge>> adj_;
+    std::vector<int> members_[3];           // vertices of group 1 (start) and 2 (goal), in order
+    int start_root_ = -1, goal_root_ = -1;
+    int add_vertex(const std::vector<float>& q);
+    void add_edge(int a, int b, float weight);
+    std::vector<int> shortest_path() const;
 };
 typedef std::shared_ptr<cRRTC> cRRTCPtr;
 typedef std::shared_ptr<RRG> RRGPtr;

@@ -1,7 +1,11 @@
 // CUDAMPLib API over the C ABI (reference: cudampl/CUDAMPLib/src).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <iostream>
+#include <limits>
+#include <queue>
 #include <stdexcept>
 
 #include "CUDAMPLib/mp_cudampl.h"
@@ -168,6 +172,12 @@ std::vector<std::vector<float>> SingleArmSpace::getJointVectorInActiveJointsFrom
     return as_states(states)->getJointStatesHost();
 }
 
+void SingleArmSpace::sampleInto(float* d_q, int n) {
+    if (n < 0) throw std::runtime_error("sample: negative count");
+    mpcpp::check(mp_sample(robot_, seed_, sample_offset_, n, d_q, nullptr), "mp_sample");
+    sample_offset_ += (unsigned long long)n;
+}
+
 BaseStatesPtr SingleArmSpace::sample(int n) {
     if (n < 0) throw std::runtime_error("sample: negative count");
     DevBuf d((size_t)n * dim_ * sizeof(float));
@@ -314,31 +324,304 @@ void cRRTC::solve(BaseTerminationPtr termination) {
     task_->setFailureReason("MeetTerminationCondition");
 }
 
+std::vector<std::vector<float>> removeRedundantVectors(const std::vector<std::vector<float>>& vec, float tol) {
+    std::vector<std::vector<float>> unique;
+    for (const auto& v : vec) {
+        bool dup = false;
+        for (const auto& u : unique) {
+            bool same = u.size() == v.size();
+            for (size_t j = 0; same && j < v.size(); j++) same = std::fabs(v[j] - u[j]) <= tol;
+            if (same) {
+                dup = true;
+                break;
+            }
+        }
+        if (!dup) unique.push_back(v);
+    }
+    return unique;
+}
+
+SingleArmTask::SingleArmTask(const std::vector<std::vector<float>>& start_joint_values, const BaseSpacePtr& goal_region,
+                             int goal_config_sampling_attempts)
+    : start_(start_joint_values) {
+    auto space = std::dynamic_pointer_cast<SingleArmSpace>(goal_region);
+    if (!space) throw std::runtime_error("SingleArmTask: the goal region must be a SingleArmSpace");
+    auto samples = space->sample(goal_config_sampling_attempts);
+    std::vector<bool> feas;
+    space->checkStates(samples, feas);
+    auto st = std::dynamic_pointer_cast<SingleArmStates>(samples);
+    st->filterStates(feas);
+    goal_ = removeRedundantVectors(st->getJointStatesHost(), 1e-5f);
+}
+
+namespace {
+// The vertices of one RRG group as a structure-of-arrays device tree [dof][capacity], the layout
+// mp_nn_argmin scans.  A host mirror keeps the rows; appends upload one span per joint row and
+// growth (doubling) re-uploads the mirror.
+struct GroupTree {
+    size_t dof;
+    int n = 0, cap = 0;
+    std::vector<float> host;
+    std::unique_ptr<DevBuf> dev;
+    explicit GroupTree(size_t d) : dof(d) {}
+    void append(const std::vector<std::vector<float>>& qs) {
+        if (qs.empty()) return;
+        const int k = (int)qs.size(), need = n + k;
+        const bool grow = need > cap;
+        if (grow) {
+            int nc = std::max(64, cap);
+            while (nc < need) nc *= 2;
+            std::vector<float> h(dof * (size_t)nc, 0.0f);
+            for (size_t j = 0; j < dof; j++)
+                std::copy(host.begin() + j * cap, host.begin() + j * cap + n, h.begin() + j * nc);
+            host.swap(h);
+            cap = nc;
+            dev.reset(new DevBuf(host.size() * sizeof(float)));
+        }
+        for (int i = 0; i < k; i++)
+            for (size_t j = 0; j < dof; j++) host[j * cap + n + i] = qs[i][j];
+        if (grow) {
+            mpcpp::check(mp_memcpy_h2d(dev->p, host.data(), host.size() * sizeof(float), nullptr), "h2d");
+        } else {
+            for (size_t j = 0; j < dof; j++)
+                mpcpp::check(mp_memcpy_h2d((float*)dev->p + j * cap + n, host.data() + j * cap + n, k * sizeof(float),
+                                           nullptr),
+                             "h2d");
+        }
+        n = need;
+    }
+    // nearest vertex position in this group of each of the nq device states d_q[nq][dof]
+    void nearest(const float* d_q, int nq, int* d_idx, std::vector<int>& out) const {
+        mpcpp::check(mp_nn_argmin((const float*)dev->p, n, cap, (int)dof, d_q, nq, d_idx, nullptr), "mp_nn_argmin");
+        out.resize(nq);
+        mpcpp::check(mp_memcpy_d2h(out.data(), d_idx, nq * sizeof(int), nullptr), "d2h");
+    }
+};
+
+void upload_rows(void* d, const std::vector<std::vector<float>>& rows, size_t offset_rows, size_t dof) {
+    if (rows.empty()) return;
+    std::vector<float> flat;
+    flat.reserve(rows.size() * dof);
+    for (const auto& r : rows) flat.insert(flat.end(), r.begin(), r.end());
+    mpcpp::check(mp_memcpy_h2d((float*)d + offset_rows * dof, flat.data(), flat.size() * sizeof(float), nullptr), "h2d");
+}
+}  // namespace
+
+int RRG::add_vertex(const std::vector<float>& q) {
+    node_.push_back(q);
+    adj_.emplace_back();
+    return (int)node_.size() - 1;
+}
+
+void RRG::add_edge(int a, int b, float weight) {
+    adj_[a].push_back({b, weight});
+    adj_[b].push_back({a, weight});
+}
+
+void RRG::setMotionTask(BaseTaskPtr task, bool get_full_path) {
+    // RRG.cpp:30-94: one root per group, joined by zero-weight edges to each start / goal state
+    task_ = task;
+    get_full_path_ = get_full_path;
+    node_.clear();
+    adj_.clear();
+    for (auto& m : members_) m.clear();
+    start_root_ = add_vertex({});
+    for (const auto& q : task->getStartStatesVector()) {
+        const int v = add_vertex(q);
+        add_edge(v, start_root_, 0.0f);
+        members_[1].push_back(v);
+    }
+    goal_root_ = add_vertex({});
+    for (const auto& q : task->getGoalStatesVector()) {
+        const int v = add_vertex(q);
+        add_edge(v, goal_root_, 0.0f);
+        members_[2].push_back(v);
+    }
+}
+
+std::vector<int> RRG::shortest_path() const {
+    // Dijkstra from the start root (RRG.cpp:454-500); vertices from start root to goal root
+    const int nv = (int)node_.size();
+    std::vector<float> dist(nv, std::numeric_limits<float>::max());
+    std::vector<int> pred(nv, -1);
+    typedef std::pair<float, int> Item;
+    std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
+    dist[start_root_] = 0.0f;
+    pq.push({0.0f, start_root_});
+    while (!pq.empty()) {
+        const Item it = pq.top();
+        pq.pop();
+        if (it.first > dist[it.second]) continue;
+        if (it.second == goal_root_) break;
+        for (const Edge& e : adj_[it.second]) {
+            const float d = it.first + e.weight;
+            if (d < dist[e.to]) {
+                dist[e.to] = d;
+                pred[e.to] = it.second;
+                pq.push({d, e.to});
+            }
+        }
+    }
+    std::vector<int> path;
+    if (pred[goal_root_] < 0) return path;
+    for (int v = goal_root_; v != -1; v = pred[v]) path.push_back(v);
+    std::reverse(path.begin(), path.end());
+    return path;
+}
+
+void RRG::getStartAndGoalGroupStates(BaseStatesPtr& start_group_states, BaseStatesPtr& goal_group_states) {
+    auto space = std::dynamic_pointer_cast<SingleArmSpace>(space_);
+    if (!space) throw std::runtime_error("RRG: space missing");
+    std::vector<std::vector<float>> a, b;
+    for (int v : members_[1]) a.push_back(node_[v]);
+    for (int v : members_[2]) b.push_back(node_[v]);
+    start_group_states = space->createStatesFromVector(a);
+    goal_group_states = space->createStatesFromVector(b);
+}
+
 void RRG::solve(BaseTerminationPtr termination) {
-    // RRG.cpp:157-232: reject invalid start/goal, accept the direct motion when it is free
     auto space = std::dynamic_pointer_cast<SingleArmSpace>(space_);
     if (!space || !task_) throw std::runtime_error("RRG: space or task missing");
-    auto starts = task_->getStartStatesVector(), goals = task_->getGoalStatesVector();
-    if (starts.empty() || goals.empty()) {
+    const size_t dof = space->getDim();
+    const float res = space->getResolution();
+    auto states_of = [&](const std::vector<int>& vs) {
+        std::vector<std::vector<float>> q;
+        for (int v : vs) q.push_back(node_[v]);
+        return q;
+    };
+    const std::vector<int> S0 = members_[1], G0 = members_[2];
+    if (S0.empty() || G0.empty()) {
         task_->setFailureReason("InvalidInput");
         return;
     }
-    std::vector<bool> f1, f2;
-    space->checkStates(space->createStatesFromVector({starts[0]}), f1);
-    space->checkStates(space->createStatesFromVector({goals[0]}), f2);
-    if (!f1[0] || !f2[0]) {
+    // RRG.cpp:144-177: some start state and some goal state must be free
+    std::vector<bool> fs, fg;
+    space->checkStates(space->createStatesFromVector(states_of(S0)), fs);
+    space->checkStates(space->createStatesFromVector(states_of(G0)), fg);
+    if (std::find(fs.begin(), fs.end(), true) == fs.end() || std::find(fg.begin(), fg.end(), true) == fg.end()) {
         task_->setFailureReason("InvalidInput");
         return;
     }
-    std::vector<bool> mf;
-    std::vector<float> mc;
-    space->checkMotions(space->createStatesFromVector({starts[0]}), space->createStatesFromVector({goals[0]}), mf, mc);
-    if (mf[0]) {
-        std::vector<std::vector<float>> path{starts[0], goals[0]};
-        task_->setSolution(get_full_path_ ? densify(path, space->getResolution()) : path);
-        return;
+    // RRG.cpp:179-232: the cheapest free direct motion between a start and a goal state wins
+    {
+        std::vector<std::vector<float>> a, b;
+        for (int s : S0)
+            for (int g : G0) {
+                a.push_back(node_[s]);
+                b.push_back(node_[g]);
+            }
+        std::vector<bool> mf;
+        std::vector<float> mc;
+        space->checkMotions(space->createStatesFromVector(a), space->createStatesFromVector(b), mf, mc);
+        int best = -1;
+        float best_cost = std::numeric_limits<float>::max();
+        for (size_t i = 0; i < mf.size(); i++)
+            if (mf[i] && mc[i] < best_cost) {
+                best_cost = mc[i];
+                best = (int)i;
+            }
+        if (best >= 0) {
+            task_->setSolution(densify({a[best], b[best]}, res));
+            return;
+        }
     }
-    cRRTC::solve(termination);
+    termination->reset();
+    const int k = sample_attempts_ > 0 ? sample_attempts_ : 1;
+    float travel[3] = {0.0f, max_travel_distance_, max_travel_distance_};
+    GroupTree tree[3] = {GroupTree(dof), GroupTree(dof), GroupTree(dof)};
+    tree[1].append(states_of(members_[1]));
+    tree[2].append(states_of(members_[2]));
+    DevBuf d_smp((size_t)k * dof * sizeof(float)), d_from((size_t)k * dof * sizeof(float));
+    DevBuf d_q0((size_t)2 * k * dof * sizeof(float)), d_q1((size_t)2 * k * dof * sizeof(float));
+    DevBuf d_idx((size_t)2 * k * sizeof(int)), d_ok((size_t)2 * k), d_len((size_t)2 * k * sizeof(float));
+    std::vector<int> idx, idx_s, idx_g;
+    std::vector<uint8_t> ok;
+    std::vector<float> h_smp, len;
+    bool solved = false;
+    for (int t = 0; t < 1000000; t++) {
+        if (termination->checkTerminationCondition()) {
+            termination->printTerminationReason();
+            task_->setFailureReason("MeetTerminationCondition");
+            break;
+        }
+        const int side = t % 2 == 0 ? 1 : 2;  // RRG.cpp:254-281: start group on even iterations
+        const std::vector<int> S = members_[1], G = members_[2];
+        const std::vector<int>& own = side == 1 ? S : G;
+        // sample, then pull each sample to within the side's travel distance of its nearest node
+        space->sampleInto((float*)d_smp.p, k);
+        tree[side].nearest((const float*)d_smp.p, k, (int*)d_idx.p, idx);
+        std::vector<std::vector<float>> from(k);
+        for (int i = 0; i < k; i++) from[i] = node_[own[idx[i]]];
+        upload_rows(d_from.p, from, 0, dof);
+        mpcpp::check(mp_steer((const float*)d_from.p, (float*)d_smp.p, k, (int)dof, travel[side], nullptr), "mp_steer");
+        mpcpp::check(mp_check_states(space->robot(), space->scene(), (const float*)d_smp.p, k, (uint8_t*)d_ok.p, nullptr,
+                                     nullptr),
+                     "mp_check_states");
+        ok.resize(k);
+        h_smp.resize((size_t)k * dof);
+        mpcpp::check(mp_memcpy_d2h(ok.data(), d_ok.p, k, nullptr), "d2h");
+        mpcpp::check(mp_memcpy_d2h(h_smp.data(), d_smp.p, h_smp.size() * sizeof(float), nullptr), "d2h");
+        std::vector<std::vector<float>> kept;
+        for (int i = 0; i < k; i++)
+            if (ok[i]) kept.emplace_back(h_smp.begin() + i * dof, h_smp.begin() + (i + 1) * dof);
+        if (kept.empty()) continue;
+        const int m = (int)kept.size();
+        // motions from each kept state to its nearest node of the start group, then of the goal group
+        upload_rows(d_q0.p, kept, 0, dof);
+        upload_rows(d_q0.p, kept, m, dof);
+        tree[1].nearest((const float*)d_q0.p, m, (int*)d_idx.p, idx_s);
+        tree[2].nearest((const float*)d_q0.p, m, (int*)d_idx.p, idx_g);
+        std::vector<std::vector<float>> to(2 * m);
+        for (int i = 0; i < m; i++) {
+            to[i] = node_[S[idx_s[i]]];
+            to[m + i] = node_[G[idx_g[i]]];
+        }
+        upload_rows(d_q1.p, to, 0, dof);
+        mpcpp::check(mp_check_edges(space->robot(), space->scene(), (const float*)d_q0.p, (const float*)d_q1.p, 2 * m,
+                                    res, (uint8_t*)d_ok.p, (float*)d_len.p, nullptr),
+                     "mp_check_edges");
+        ok.resize(2 * m);
+        len.resize(2 * m);
+        mpcpp::check(mp_memcpy_d2h(ok.data(), d_ok.p, 2 * m, nullptr), "d2h");
+        mpcpp::check(mp_memcpy_d2h(len.data(), d_len.p, 2 * m * sizeof(float), nullptr), "d2h");
+        int reached[3] = {0, 0, 0};
+        for (int i = 0; i < m; i++) {
+            reached[1] += ok[i] != 0;
+            reached[2] += ok[m + i] != 0;
+        }
+        // RRG.cpp:371-398: no state reached the side's group: halve its travel distance and
+        // stay on that side (the loop index skips one)
+        if (reached[side] == 0) {
+            const double half = travel[side] * 0.5;
+            travel[side] = half > 0.1 ? (float)half : (float)0.1;
+            t++;
+            continue;
+        }
+        travel[side] = max_travel_distance_;
+        std::vector<std::vector<float>> joined[3];
+        for (int i = 0; i < m; i++) {
+            const bool to_s = ok[i] != 0, to_g = ok[m + i] != 0;
+            if (!to_s && !to_g) continue;
+            const int v = add_vertex(kept[i]);
+            if (to_s) add_edge(v, S[idx_s[i]], len[i]);
+            if (to_g) add_edge(v, G[idx_g[i]], len[m + i]);
+            if (to_s && to_g) {
+                solved = true;  // the vertex joins the two groups
+            } else {
+                const int g = to_s ? 1 : 2;
+                members_[g].push_back(v);
+                joined[g].push_back(kept[i]);
+            }
+        }
+        tree[1].append(joined[1]);
+        tree[2].append(joined[2]);
+        if (solved) break;
+    }
+    if (!solved) return;
+    std::vector<std::vector<float>> waypoints;
+    for (int v : shortest_path())
+        if (v != start_root_ && v != goal_root_) waypoints.push_back(node_[v]);
+    task_->setSolution(get_full_path_ ? densify(waypoints, res) : waypoints);
 }
 
 }  // namespace CUDAMPLib

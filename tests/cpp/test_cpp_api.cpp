@@ -127,6 +127,63 @@ int main(int argc, char** argv) {
     planner->solve(std::make_shared<CUDAMPLib::TimeoutTermination>(10.0f));
     std::printf("rrg_solved %d %s\n", (int)task->hasSolution(), task->getFailureReason().c_str());
     print_path("rrg_path", task->getSolution());
+    CUDAMPLib::BaseStatesPtr gs, gg;
+    planner->getStartAndGoalGroupStates(gs, gg);
+    std::printf("rrg_groups %d %d\n", gs->getNumOfStates(), gg->getNumOfStates());
+
+    // a goal region of two states, the first inside the wall: the path ends at the free one
+    std::vector<float> in_wall(dof, 0.0f);
+    auto task_multi = std::make_shared<CUDAMPLib::SingleArmTask>(std::vector<std::vector<float>>{start},
+                                                                 std::vector<std::vector<float>>{in_wall, goal});
+    auto planner_multi = std::make_shared<CUDAMPLib::RRG>(space);
+    planner_multi->setSampleAttemptsInEachIteration(60);
+    planner_multi->setMotionTask(task_multi, false);
+    planner_multi->solve(std::make_shared<CUDAMPLib::TimeoutTermination>(10.0f));
+    std::printf("rrg_multi_solved %d\n", (int)task_multi->hasSolution());
+    print_path("rrg_multi_path", task_multi->getSolution());
+
+    // a goal region: a space whose bounds box the goal (SingleArmTask's region constructor)
+    std::vector<float> rlo = lo, rhi = hi;
+    for (int i = 0, a = 0; i < L; i++)
+        if (act[i]) {
+            rlo[i] = goal[a] - 0.05f;
+            rhi[i] = goal[a] + 0.05f;
+            a++;
+        }
+    auto rbound = std::make_shared<CUDAMPLib::BoundaryConstraint>("bound", rlo, rhi, act);
+    std::vector<CUDAMPLib::BaseConstraintPtr> rcons{env, self, rbound};
+    auto region = std::make_shared<CUDAMPLib::SingleArmSpace>(dof, rcons, jt, poses, axes, par, smap, spos, rad, act,
+                                                              rlo, rhi, def, names, 0.02f);
+    auto task_region = std::make_shared<CUDAMPLib::SingleArmTask>(std::vector<std::vector<float>>{start}, region, 20);
+    auto planner_region = std::make_shared<CUDAMPLib::RRG>(space);
+    planner_region->setMotionTask(task_region, false);
+    planner_region->solve(std::make_shared<CUDAMPLib::TimeoutTermination>(10.0f));
+    std::printf("rrg_region %d %d\n", (int)task_region->getGoalStatesVector().size(), (int)task_region->hasSolution());
+    print_path("rrg_region_path", task_region->getSolution());
+
+    // a free direct motion is returned at once, densified to the space's resolution
+    std::vector<float> near_goal = start;
+    near_goal[1] += 0.3f;
+    auto task_direct = std::make_shared<CUDAMPLib::SingleArmTask>(std::vector<std::vector<float>>{start},
+                                                                  std::vector<std::vector<float>>{near_goal});
+    auto planner_direct = std::make_shared<CUDAMPLib::RRG>(space);
+    planner_direct->setMotionTask(task_direct, false);
+    planner_direct->solve(std::make_shared<CUDAMPLib::StepTermination>(0));
+    print_path("rrg_direct_path", task_direct->getSolution());
+
+    // a start inside the wall is rejected; a blocked query with no steps left terminates
+    auto task_bad = std::make_shared<CUDAMPLib::SingleArmTask>(std::vector<std::vector<float>>{in_wall},
+                                                               std::vector<std::vector<float>>{goal});
+    auto planner_bad = std::make_shared<CUDAMPLib::RRG>(space);
+    planner_bad->setMotionTask(task_bad, false);
+    planner_bad->solve(std::make_shared<CUDAMPLib::StepTermination>(100));
+    std::printf("rrg_invalid %d %s\n", (int)task_bad->hasSolution(), task_bad->getFailureReason().c_str());
+    auto task_s0 = std::make_shared<CUDAMPLib::SingleArmTask>(std::vector<std::vector<float>>{start},
+                                                              std::vector<std::vector<float>>{goal});
+    auto planner_s0 = std::make_shared<CUDAMPLib::RRG>(space);
+    planner_s0->setMotionTask(task_s0, false);
+    planner_s0->solve(std::make_shared<CUDAMPLib::StepTermination>(0));
+    std::printf("rrg_step0 %d %s\n", (int)task_s0->hasSolution(), task_s0->getFailureReason().c_str());
 
     auto task0 = std::make_shared<CUDAMPLib::SingleArmTask>(std::vector<std::vector<float>>{start},
                                                             std::vector<std::vector<float>>{goal});

@@ -53,8 +53,33 @@ def test_cpp_api(tmp_path):
     assert res["space_feasible"] == ["1", "1"]
     assert res["space_motion"][0] == "0" and abs(float(res["space_motion"][1]) - 1.0) < 1e-6
     assert res["space_samples"][0] == "64"
+    # RRG (roadmap between the start and goal groups): waypoints joined by motions that pass the
+    # same checker at the space's resolution, start and goal at the ends, around the wall.  The
+    # roadmap checks each motion from the new state toward its neighbour, so a path edge passes
+    # in one of its two directions (interpolated states differ by rounding between them).
+    def roadmap_edges_free(w):
+        fwd = oracle.edges(ro, so, w[:-1], w[1:], 0.02)[0]
+        bwd = oracle.edges(ro, so, w[1:], w[:-1], 0.02)[0]
+        return bool(np.all((fwd != 0) | (bwd != 0)))
+
     assert res["rrg_solved"][0] == "1"
     q = path("rrg_path")
-    assert np.array_equal(q[0], start) and np.array_equal(q[-1], goal)
-    assert oracle.validate_path(ro, so, q, 0.005)[0]
+    assert len(q) >= 3 and np.array_equal(q[0], start) and np.array_equal(q[-1], goal)
+    assert roadmap_edges_free(q)
+    ns, ng = (int(x) for x in res["rrg_groups"])
+    assert ns >= 1 and ng >= 1 and ns + ng >= len(q) - 1
+    assert res["rrg_multi_solved"] == ["1"]
+    qm = path("rrg_multi_path")
+    assert np.array_equal(qm[0], start) and np.array_equal(qm[-1], goal)
+    assert roadmap_edges_free(qm)
+    ngoal, solved = (int(x) for x in res["rrg_region"])
+    assert 1 <= ngoal <= 20 and solved == 1
+    qr = path("rrg_region_path")
+    assert np.array_equal(qr[0], start) and np.all(np.abs(qr[-1] - np.float32(goal)) <= 0.05 + 1e-6)
+    assert roadmap_edges_free(qr)
+    qd = path("rrg_direct_path")
+    assert np.array_equal(qd[0], start) and np.allclose(qd[-1], [-0.5, 0.3, 0.0])
+    assert np.all(np.linalg.norm(np.diff(qd, axis=0), axis=1) <= 0.02 + 1e-6) and len(qd) >= 15
+    assert res["rrg_invalid"] == ["0", "InvalidInput"]
+    assert res["rrg_step0"] == ["0", "MeetTerminationCondition"]
     assert res["step0"] == ["0", "MeetTerminationCondition"]
