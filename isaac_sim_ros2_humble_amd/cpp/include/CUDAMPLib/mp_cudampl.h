@@ -126,6 +126,35 @@ class SingleArmStates : public BaseStates {
 };
 typedef std::shared_ptr<SingleArmStates> SingleArmStatesPtr;
 
+// Node store of the planners (States.h:218-283, SingleArmStates.h:185-221): joint values of
+// every state added, addressed by the index add_states returned.  Nearest neighbours run on the
+// device (mp_nn_argmin over the group's states, ties to the earlier member of the group).
+class BaseStateManager {
+  public:
+    virtual ~BaseStateManager() {}
+    virtual void clear() = 0;
+    virtual std::vector<int> add_states(const BaseStatesPtr& states) = 0;
+    int get_num_of_states() const { return num_of_states_; }
+    // the k nearest members of each group for each query (ascending distance; fewer when a
+    // group is smaller), returns k
+    virtual int find_k_nearest_neighbors(int k, const BaseStatesPtr& query_states,
+                                         const std::vector<std::vector<int>>& group_indexs,
+                                         std::vector<std::vector<int>>& neighbors_index) = 0;
+    // neighbors_index[query][group] = the nearest member of group_indexs[group]
+    virtual void find_the_nearest_neighbors(const BaseStatesPtr& query_states,
+                                            const std::vector<std::vector<int>>& group_indexs,
+                                            std::vector<std::vector<int>>& neighbors_index) = 0;
+    virtual BaseStatesPtr get_states(const std::vector<int>& states_index) = 0;
+    virtual BaseStatesPtr concatinate_states(const std::vector<BaseStatesPtr>& states) = 0;
+    // states[i] <- within max_distance of stored state nearest_neighbors_indexes[i]
+    virtual void interpolateToStates(const BaseStatesPtr& states, std::vector<int>& nearest_neighbors_indexes,
+                                     float max_distance) = 0;
+
+  protected:
+    int num_of_states_ = 0;
+};
+typedef std::shared_ptr<BaseStateManager> BaseStateManagerPtr;
+
 // ------------------------------------------------------------------------------------ spaces
 class BaseSpace {
   public:
@@ -137,6 +166,7 @@ class BaseSpace {
                               std::vector<bool>& motion_feasibility, std::vector<float>& motion_costs) = 0;
     virtual void interpolate(const BaseStatesPtr& from_states, const BaseStatesPtr& to_states, float max_distance) = 0;
     virtual BaseStatesPtr getPathFromWaypoints(const BaseStatesPtr& waypoints) = 0;
+    virtual BaseStateManagerPtr createStateManager() = 0;
     size_t getDim() const { return dim_; }
 
   protected:
@@ -165,6 +195,7 @@ class SingleArmSpace : public BaseSpace {
                       std::vector<float>& motion_costs) override;
     void interpolate(const BaseStatesPtr& from_states, const BaseStatesPtr& to_states, float max_distance) override;
     BaseStatesPtr getPathFromWaypoints(const BaseStatesPtr& waypoints) override;
+    BaseStateManagerPtr createStateManager() override;
     BaseStatesPtr createStatesFromVector(const std::vector<std::vector<float>>& joint_values);
     BaseStatesPtr createStatesFromVectorFull(const std::vector<std::vector<float>>& joint_values);
     std::vector<std::vector<float>> getJointVectorInActiveJointsFromStates(const BaseStatesPtr& states);
@@ -185,6 +216,27 @@ class SingleArmSpace : public BaseSpace {
     mp_scene* scene_ = nullptr;
 };
 typedef std::shared_ptr<SingleArmSpace> SingleArmSpacePtr;
+
+class SingleArmStateManager : public BaseStateManager {
+  public:
+    explicit SingleArmStateManager(SingleArmSpace* space) : space_(space) {}
+    void clear() override;
+    std::vector<int> add_states(const BaseStatesPtr& states) override;
+    int find_k_nearest_neighbors(int k, const BaseStatesPtr& query_states,
+                                 const std::vector<std::vector<int>>& group_indexs,
+                                 std::vector<std::vector<int>>& neighbors_index) override;
+    void find_the_nearest_neighbors(const BaseStatesPtr& query_states, const std::vector<std::vector<int>>& group_indexs,
+                                    std::vector<std::vector<int>>& neighbors_index) override;
+    BaseStatesPtr get_states(const std::vector<int>& states_index) override;
+    BaseStatesPtr concatinate_states(const std::vector<BaseStatesPtr>& states) override;
+    void interpolateToStates(const BaseStatesPtr& states, std::vector<int>& nearest_neighbors_indexes,
+                             float max_distance) override;
+
+  private:
+    SingleArmSpace* space_;  // the space that created the manager; it must outlive the manager
+    std::vector<std::vector<float>> q_;
+};
+typedef std::shared_ptr<SingleArmStateManager> SingleArmStateManagerPtr;
 
 // --------------------------------------------------------------------------------- util
 std::vector<std::vector<float>> interpolateVectors(const std::vector<float>& v1, const std::vector<float>& v2,
@@ -321,6 +373,8 @@ class RRG : public BasePlanner {
     void setSampleAttemptsInEachIteration(int n) { sample_attempts_ = n; }
     // states of the start group and of the goal group (RRG.cpp:531-544)
     void getStartAndGoalGroupStates(BaseStatesPtr& start_group_states, BaseStatesPtr& goal_group_states);
+    // the roadmap's states (start states, goal states, then added states in order of addition)
+    BaseStateManagerPtr getStateManager();
 
   private:
     struct Edge {

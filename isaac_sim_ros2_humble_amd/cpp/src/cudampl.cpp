@@ -256,6 +256,108 @@ BaseStatesPtr SingleArmSpace::getPathFromWaypoints(const BaseStatesPtr& waypoint
     return createStatesFromVectorFull(path);
 }
 
+BaseStateManagerPtr SingleArmSpace::createStateManager() {
+    return std::make_shared<SingleArmStateManager>(this);
+}
+
+// ----------------------------------------------------------------------------- state manager
+void SingleArmStateManager::clear() {
+    q_.clear();
+    num_of_states_ = 0;
+}
+
+std::vector<int> SingleArmStateManager::add_states(const BaseStatesPtr& states) {
+    auto st = as_states(states);
+    std::vector<int> idx;
+    for (const auto& q : st->getJointStatesHost()) {
+        idx.push_back((int)q_.size());
+        q_.push_back(q);
+    }
+    num_of_states_ = (int)q_.size();
+    return idx;
+}
+
+BaseStatesPtr SingleArmStateManager::get_states(const std::vector<int>& states_index) {
+    std::vector<std::vector<float>> q;
+    for (int i : states_index) {
+        if (i < 0 || i >= (int)q_.size()) throw std::runtime_error("get_states: index out of range");
+        q.push_back(q_[i]);
+    }
+    return space_->createStatesFromVector(q);
+}
+
+BaseStatesPtr SingleArmStateManager::concatinate_states(const std::vector<BaseStatesPtr>& states) {
+    std::vector<std::vector<float>> q;
+    for (const auto& s : states) {
+        auto h = as_states(s)->getJointStatesHost();
+        q.insert(q.end(), h.begin(), h.end());
+    }
+    return space_->createStatesFromVector(q);
+}
+
+void SingleArmStateManager::find_the_nearest_neighbors(const BaseStatesPtr& query_states,
+                                                       const std::vector<std::vector<int>>& group_indexs,
+                                                       std::vector<std::vector<int>>& neighbors_index) {
+    // SingleArmStates.cu:1265-1431: per query and group, argmin of the squared L2 distance over
+    // the active joints; the group's members are gathered into a SoA tree for mp_nn_argmin
+    auto qs = as_states(query_states);
+    const int nq = qs->getNumOfStates();
+    const size_t dof = space_->getDim();
+    neighbors_index.assign(nq, std::vector<int>(group_indexs.size(), -1));
+    if (nq == 0) return;
+    DevBuf d_idx((size_t)nq * sizeof(int));
+    std::vector<int> pos(nq);
+    for (size_t g = 0; g < group_indexs.size(); g++) {
+        const auto& members = group_indexs[g];
+        if (members.empty()) throw std::runtime_error("find_the_nearest_neighbors: empty group");
+        const int n = (int)members.size();
+        std::vector<float> soa(dof * (size_t)n);
+        for (int i = 0; i < n; i++) {
+            if (members[i] < 0 || members[i] >= (int)q_.size()) throw std::runtime_error("group index out of range");
+            for (size_t j = 0; j < dof; j++) soa[j * n + i] = q_[members[i]][j];
+        }
+        DevBuf d_tree(soa.size() * sizeof(float));
+        mpcpp::check(mp_memcpy_h2d(d_tree.p, soa.data(), soa.size() * sizeof(float), nullptr), "h2d");
+        mpcpp::check(mp_nn_argmin((const float*)d_tree.p, n, n, (int)dof, qs->getJointStatesCuda(), nq, (int*)d_idx.p,
+                                  nullptr),
+                     "mp_nn_argmin");
+        mpcpp::check(mp_memcpy_d2h(pos.data(), d_idx.p, nq * sizeof(int), nullptr), "d2h");
+        for (int i = 0; i < nq; i++) neighbors_index[i][g] = members[pos[i]];
+    }
+}
+
+int SingleArmStateManager::find_k_nearest_neighbors(int k, const BaseStatesPtr& query_states,
+                                                    const std::vector<std::vector<int>>& group_indexs,
+                                                    std::vector<std::vector<int>>& neighbors_index) {
+    // unused by the planners (States.h:238-245); host selection by (distance, group order)
+    auto qs = as_states(query_states)->getJointStatesHost();
+    neighbors_index.assign(qs.size(), {});
+    for (size_t i = 0; i < qs.size(); i++)
+        for (const auto& members : group_indexs) {
+            std::vector<std::pair<float, int>> d;
+            for (size_t m = 0; m < members.size(); m++) {
+                float acc = 0.0f;
+                for (size_t j = 0; j < qs[i].size(); j++) {
+                    const float df = qs[i][j] - q_[members[m]][j];
+                    acc = std::fma(df, df, acc);
+                }
+                d.push_back({acc, (int)m});
+            }
+            const size_t kk = std::min(d.size(), (size_t)std::max(k, 0));
+            std::partial_sort(d.begin(), d.begin() + kk, d.end());
+            for (size_t r = 0; r < kk; r++) neighbors_index[i].push_back(members[d[r].second]);
+        }
+    return k;
+}
+
+void SingleArmStateManager::interpolateToStates(const BaseStatesPtr& states, std::vector<int>& nearest_neighbors_indexes,
+                                                float max_distance) {
+    // SingleArmStates.cu:1488-1557: pull each state to within max_distance of its neighbour
+    auto from = get_states(nearest_neighbors_indexes);
+    if (from->getNumOfStates() != states->getNumOfStates()) throw std::runtime_error("interpolateToStates: size mismatch");
+    space_->interpolate(from, states, max_distance);
+}
+
 // util.cpp:205-248
 std::vector<std::vector<float>> interpolateVectors(const std::vector<float>& v1, const std::vector<float>& v2,
                                                    float resolution) {
@@ -467,6 +569,17 @@ std::vector<int> RRG::shortest_path() const {
     for (int v = goal_root_; v != -1; v = pred[v]) path.push_back(v);
     std::reverse(path.begin(), path.end());
     return path;
+}
+
+BaseStateManagerPtr RRG::getStateManager() {
+    auto space = std::dynamic_pointer_cast<SingleArmSpace>(space_);
+    if (!space) throw std::runtime_error("RRG: space missing");
+    auto m = space->createStateManager();
+    std::vector<std::vector<float>> q;
+    for (int v = 0; v < (int)node_.size(); v++)
+        if (v != start_root_ && v != goal_root_) q.push_back(node_[v]);
+    m->add_states(space->createStatesFromVector(q));
+    return m;
 }
 
 void RRG::getStartAndGoalGroupStates(BaseStatesPtr& start_group_states, BaseStatesPtr& goal_group_states) {

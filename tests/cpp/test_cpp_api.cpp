@@ -142,6 +142,34 @@ int main(int argc, char** argv) {
     std::printf("rrg_multi_solved %d\n", (int)task_multi->hasSolution());
     print_path("rrg_multi_path", task_multi->getSolution());
 
+    // state manager (SingleArmStateManager): add, nearest per group, gather, steer toward a node
+    auto mgr = space->createStateManager();
+    std::vector<std::vector<float>> stored{{0.1f, 0.2f, 0.3f}, {-0.4f, 0.5f, 0.0f}, {0.9f, -0.9f, 0.1f},
+                                           {0.0f, 0.0f, 0.8f}, {-0.7f, -0.2f, -0.6f}};
+    auto i1 = mgr->add_states(space->createStatesFromVector({stored[0], stored[1], stored[2]}));
+    auto i2 = mgr->add_states(space->createStatesFromVector({stored[3], stored[4]}));
+    std::printf("mgr_add %d %d %d %d %d %d\n", i1[0], i1[1], i1[2], i2[0], i2[1], mgr->get_num_of_states());
+    std::vector<std::vector<float>> queries{{0.0f, 0.3f, 0.2f}, {0.8f, -0.5f, 0.5f}, {-0.6f, 0.1f, -0.3f}};
+    std::vector<std::vector<int>> nn;
+    mgr->find_the_nearest_neighbors(space->createStatesFromVector(queries), {{0, 1, 2}, {3, 4}}, nn);
+    std::printf("mgr_nn");
+    for (auto& r : nn) std::printf(" %d %d", r[0], r[1]);
+    std::printf("\n");
+    std::vector<std::vector<int>> knn;
+    mgr->find_k_nearest_neighbors(2, space->createStatesFromVector(queries), {{0, 1, 2, 3, 4}}, knn);
+    std::printf("mgr_knn");
+    for (auto& r : knn) std::printf(" %d %d", r[0], r[1]);
+    std::printf("\n");
+    auto got = mgr->get_states({4, 0});
+    print_path("mgr_get", space->getJointVectorInActiveJointsFromStates(got));
+    auto cat = mgr->concatinate_states({got, space->createStatesFromVector(queries)});
+    std::printf("mgr_cat %d\n", cat->getNumOfStates());
+    auto far = space->createStatesFromVector({{0.9f, 0.9f, 0.9f}});
+    std::vector<int> to0{0};
+    mgr->interpolateToStates(far, to0, 0.1f);
+    print_path("mgr_steer", space->getJointVectorInActiveJointsFromStates(far));
+    std::printf("rrg_manager %d\n", planner->getStateManager()->get_num_of_states());
+
     // a goal region: a space whose bounds box the goal (SingleArmTask's region constructor)
     std::vector<float> rlo = lo, rhi = hi;
     for (int i = 0, a = 0; i < L; i++)

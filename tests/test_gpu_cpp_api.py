@@ -72,6 +72,22 @@ def test_cpp_api(tmp_path):
     qm = path("rrg_multi_path")
     assert np.array_equal(qm[0], start) and np.array_equal(qm[-1], goal)
     assert roadmap_edges_free(qm)
+    # state manager: indices in order of addition, nearest member per group (brute force),
+    # gathers, and steering to within the travel distance of a stored state
+    assert res["mgr_add"] == ["0", "1", "2", "3", "4", "5"]
+    stored = np.array([[0.1, 0.2, 0.3], [-0.4, 0.5, 0.0], [0.9, -0.9, 0.1], [0.0, 0.0, 0.8], [-0.7, -0.2, -0.6]],
+                      np.float32)
+    queries = np.array([[0.0, 0.3, 0.2], [0.8, -0.5, 0.5], [-0.6, 0.1, -0.3]], np.float32)
+    d2 = ((queries[:, None, :].astype(np.float64) - stored[None, :, :]) ** 2).sum(-1)
+    want = [[int(np.argmin(d2[i, :3])), 3 + int(np.argmin(d2[i, 3:]))] for i in range(3)]
+    assert [int(x) for x in res["mgr_nn"]] == [v for r in want for v in r]
+    want_k = [list(np.argsort(d2[i])[:2]) for i in range(3)]
+    assert [int(x) for x in res["mgr_knn"]] == [int(v) for r in want_k for v in r]
+    assert np.array_equal(path("mgr_get"), stored[[4, 0]])
+    assert res["mgr_cat"] == ["5"]
+    st = path("mgr_steer")[0]
+    assert abs(np.linalg.norm(st.astype(np.float64) - stored[0]) - 0.1) < 1e-5
+    assert int(res["rrg_manager"][0]) >= ns + ng
     ngoal, solved = (int(x) for x in res["rrg_region"])
     assert 1 <= ngoal <= 20 and solved == 1
     qr = path("rrg_region_path")
