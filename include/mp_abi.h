@@ -140,6 +140,13 @@ int mp_scene_destroy(mp_scene* scene);
  * fused, no link poses written. */
 int mp_fk_spheres(const mp_robot* robot, const float* d_q, int64_t n, float* d_centres, void* stream);
 
+/* Link poses of n configurations: poses [n][L][16], the pose of every link in the base frame as
+ * a row-major 4x4 with last row (0, 0, 0, 1); L is mp_robot_info's num_links, in the robot
+ * description's link order.  Replaces kin_forward_nvrtc_kernel and
+ * SingleArmStates::calculateForwardKinematicsNvrtv (KernelGenerator.h:222-406,
+ * SingleArmStates.cu:752-768). */
+int mp_fk_link_poses(const mp_robot* robot, const float* d_q, int64_t n, float* d_poses, void* stream);
+
 /* Collision verdicts of n configurations: verdict[i] = 1 iff every constraint cost is 0
  * (self pairs, all obstacle types, joint bounds).  Early-out per wave.  d_centres optional.
  * Replaces states->update(); space->checkStates(states, feasible)

@@ -110,18 +110,34 @@ class SingleArmStates : public BaseStates {
     SingleArmStates(const std::vector<std::vector<float>>& active_values, const std::vector<bool>& active_map,
                     const std::vector<float>& defaults);
     ~SingleArmStates() override;
-    void update() {}  // FK is fused into every check kernel
+    // Forward kinematics of every state: link poses and sphere centres into device buffers
+    // (SingleArmStates::update, SingleArmStates.cu:696-723).  The collision checks fuse FK and
+    // need no update; the pose and centre getters run it on first use after a change.
+    void update();
     float* getJointStatesCuda() { return d_q_; }  // [n][dof] active joints
     std::vector<std::vector<float>> getJointStatesHost() const { return q_; }
     std::vector<std::vector<float>> getJointStatesFullHost() const;
     void filterStates(const std::vector<bool>& keep);
+    float* getLinkPosesInBaseLinkCuda();                // [n][L][16] row-major 4x4 per link
+    float* getSelfCollisionSpheresPosInBaseLinkCuda();  // [n][S][3]
+    std::vector<std::vector<std::vector<float>>> getSelfCollisionSpheresPosInBaseLinkHost();
+    std::vector<std::vector<mpx::Isometry3d>> getLinkPosesInBaseLinkHost();
+    std::vector<mpx::Isometry3d> getLinkPoseInBaseLinkHost(const std::string& link_name);
 
   private:
     std::vector<std::vector<float>> q_;
     std::vector<bool> active_;
     std::vector<float> defaults_;
     float* d_q_ = nullptr;
+    // the robot of the space that created these states (the space outlives its states)
+    const mp_robot* robot_ = nullptr;
+    std::vector<std::string> link_names_;
+    int num_spheres_ = 0;
+    float* d_poses_ = nullptr;
+    float* d_centres_ = nullptr;
+    bool fk_current_ = false;
     void upload();
+    void free_fk();
     friend class SingleArmSpace;
 };
 typedef std::shared_ptr<SingleArmStates> SingleArmStatesPtr;
@@ -209,6 +225,8 @@ class SingleArmSpace : public BaseSpace {
   private:
     std::vector<float> lower_bound, upper_bound, defaults_;
     std::vector<bool> active_;
+    std::vector<std::string> link_names_;
+    int num_spheres_ = 0;
     float resolution_;
     unsigned long long sample_offset_ = 0;
     unsigned long long seed_ = 0x5eed;

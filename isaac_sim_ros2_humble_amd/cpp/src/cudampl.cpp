@@ -55,9 +55,77 @@ SingleArmStates::SingleArmStates(const std::vector<std::vector<float>>& active_v
 
 SingleArmStates::~SingleArmStates() {
     if (d_q_) mp_device_free(d_q_);
+    free_fk();
+}
+
+void SingleArmStates::free_fk() {
+    if (d_poses_) mp_device_free(d_poses_);
+    if (d_centres_) mp_device_free(d_centres_);
+    d_poses_ = d_centres_ = nullptr;
+    fk_current_ = false;
+}
+
+void SingleArmStates::update() {
+    if (!robot_) throw std::runtime_error("update: states are not bound to a space");
+    const int n = num_of_states_, L = (int)link_names_.size();
+    if (!d_poses_) mpcpp::check(mp_device_alloc((void**)&d_poses_, std::max<size_t>((size_t)n * L * 64, 4)), "alloc");
+    if (!d_centres_)
+        mpcpp::check(mp_device_alloc((void**)&d_centres_, std::max<size_t>((size_t)n * num_spheres_ * 12, 4)), "alloc");
+    if (n > 0) {
+        mpcpp::check(mp_fk_link_poses(robot_, d_q_, n, d_poses_, nullptr), "mp_fk_link_poses");
+        mpcpp::check(mp_fk_spheres(robot_, d_q_, n, d_centres_, nullptr), "mp_fk_spheres");
+        mpcpp::check(mp_stream_sync(nullptr), "sync");
+    }
+    fk_current_ = true;
+}
+
+float* SingleArmStates::getLinkPosesInBaseLinkCuda() {
+    if (!fk_current_) update();
+    return d_poses_;
+}
+
+float* SingleArmStates::getSelfCollisionSpheresPosInBaseLinkCuda() {
+    if (!fk_current_) update();
+    return d_centres_;
+}
+
+std::vector<std::vector<std::vector<float>>> SingleArmStates::getSelfCollisionSpheresPosInBaseLinkHost() {
+    const float* d = getSelfCollisionSpheresPosInBaseLinkCuda();
+    const int n = num_of_states_, S = num_spheres_;
+    std::vector<float> h((size_t)n * S * 3);
+    if (!h.empty()) mpcpp::check(mp_memcpy_d2h(h.data(), d, h.size() * sizeof(float), nullptr), "d2h");
+    std::vector<std::vector<std::vector<float>>> out(n, std::vector<std::vector<float>>(S));
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < S; k++) out[i][k].assign(h.begin() + ((size_t)i * S + k) * 3, h.begin() + ((size_t)i * S + k + 1) * 3);
+    return out;
+}
+
+std::vector<std::vector<mpx::Isometry3d>> SingleArmStates::getLinkPosesInBaseLinkHost() {
+    const float* d = getLinkPosesInBaseLinkCuda();
+    const int n = num_of_states_, L = (int)link_names_.size();
+    std::vector<float> h((size_t)n * L * 16);
+    if (!h.empty()) mpcpp::check(mp_memcpy_d2h(h.data(), d, h.size() * sizeof(float), nullptr), "d2h");
+    std::vector<std::vector<mpx::Isometry3d>> out(n, std::vector<mpx::Isometry3d>(L));
+    for (int i = 0; i < n; i++)
+        for (int l = 0; l < L; l++)
+            for (int r = 0; r < 4; r++)
+                for (int c = 0; c < 4; c++) out[i][l].matrix()(r, c) = h[((size_t)i * L + l) * 16 + 4 * r + c];
+    return out;
+}
+
+std::vector<mpx::Isometry3d> SingleArmStates::getLinkPoseInBaseLinkHost(const std::string& link_name) {
+    // SingleArmStates.cu getLinkPoseInBaseLinkHost: the named link's pose in every state
+    const auto it = std::find(link_names_.begin(), link_names_.end(), link_name);
+    if (it == link_names_.end()) throw std::runtime_error("getLinkPoseInBaseLinkHost: unknown link " + link_name);
+    const int l = (int)(it - link_names_.begin());
+    auto all = getLinkPosesInBaseLinkHost();
+    std::vector<mpx::Isometry3d> out;
+    for (auto& p : all) out.push_back(p[l]);
+    return out;
 }
 
 void SingleArmStates::upload() {
+    free_fk();
     if (d_q_) mp_device_free(d_q_);
     d_q_ = nullptr;
     size_t dof = q_.empty() ? 0 : q_[0].size();
@@ -140,6 +208,10 @@ SingleArmSpace::SingleArmSpace(size_t dim, const std::vector<BaseConstraintPtr>&
                          link_names.size(), enables);
     if (a.dof != (int)dim) throw std::runtime_error("SingleArmSpace: dim does not match the active joints");
     mpcpp::check(mp_robot_create(&a.desc, &robot_), "mp_robot_create");
+    int rdof = 0, rpairs = 0, rlinks = 0;
+    mpcpp::check(mp_robot_info(robot_, &rdof, &num_spheres_, &rpairs, &rlinks), "mp_robot_info");
+    link_names_ = link_names;
+    link_names_.resize(rlinks);
     mpcpp::check(mp_scene_create(s.data(), (int)(s.size() / 4), b.data(), (int)(b.size() / 18), c.data(),
                                  (int)(c.size() / 14), nullptr, 0, &scene_),
                  "mp_scene_create");
@@ -153,7 +225,11 @@ SingleArmSpace::~SingleArmSpace() {
 BaseStatesPtr SingleArmSpace::createStatesFromVector(const std::vector<std::vector<float>>& joint_values) {
     for (const auto& r : joint_values)
         if (r.size() != dim_) throw std::runtime_error("createStatesFromVector: expected active joint values");
-    return std::make_shared<SingleArmStates>(joint_values, active_, defaults_);
+    auto st = std::make_shared<SingleArmStates>(joint_values, active_, defaults_);
+    st->robot_ = robot_;
+    st->link_names_ = link_names_;
+    st->num_spheres_ = num_spheres_;
+    return st;
 }
 
 BaseStatesPtr SingleArmSpace::createStatesFromVectorFull(const std::vector<std::vector<float>>& joint_values) {

@@ -272,13 +272,12 @@ constexpr int kXform = 18, kSelf = 9, kSph = 11, kBox = 34, kCyl = 35, kCap = 28
 
 }  // namespace
 
-std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
-    // ---------------------------------------------------------------- forward kinematics
-    Emitter E;
+// Link frames of every link (KernelGenerator.h:254-348 composition order), emitted into E with
+// constant folding; returns the symbolic frames.
+std::vector<Aff> emit_frames(Emitter& E, const RobotModel& m, int& fk_flops) {
     std::vector<Aff> T(m.L);
     T[0] = identity();
     int ntrig = 0;
-    int fk_flops = 0;
     for (int i = 1; i < m.L; i++) {
         const Aff& P = T[m.parent[i]];
         Aff O;
@@ -322,6 +321,31 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
         } else {
             T[i] = P;
         }
+    }
+    return T;
+}
+
+std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
+    // ---------------------------------------------------------------- forward kinematics
+    Emitter E;
+    int fk_flops = 0;
+    const std::vector<Aff> T = emit_frames(E, m, fk_flops);
+    // link poses [L][16], row-major 4x4 with the last row (0, 0, 0, 1)
+    std::string poses_body;
+    {
+        Emitter EP;
+        int unused = 0;
+        const std::vector<Aff> TP = emit_frames(EP, m, unused);
+        for (int i = 0; i < m.L; i++) {
+            for (int a = 0; a < 3; a++) {
+                for (int b = 0; b < 3; b++)
+                    EP.os << EP.ind << "P[" << 16 * i + 4 * a + b << "] = " << EP.ref(TP[i].R[3 * a + b]) << ";\n";
+                EP.os << EP.ind << "P[" << 16 * i + 4 * a + 3 << "] = " << EP.ref(TP[i].t[a]) << ";\n";
+            }
+            EP.os << EP.ind << "P[" << 16 * i + 12 << "] = 0.0f; P[" << 16 * i + 13 << "] = 0.0f; P[" << 16 * i + 14
+                  << "] = 0.0f; P[" << 16 * i + 15 << "] = 1.0f;\n";
+        }
+        poses_body = EP.os.str();
     }
     const std::vector<SphereLink> SL = sphere_links(m);
     const int NL = (int)SL.size();
@@ -625,6 +649,8 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
     o << "    static constexpr int NF = " << NL * 15 << ";\n";
     o << "    static constexpr unsigned FK_WORK = " << fk_flops << "u;\n";
     o << "    static __device__ __forceinline__ void fk(const float* q, float* F) {\n" << fk_body << "    }\n";
+    o << "    static constexpr int NLINK = " << m.L << ";\n";
+    o << "    static __device__ __forceinline__ void poses(const float* q, float* P) {\n" << poses_body << "    }\n";
     o << "    static __device__ __forceinline__ void write_centres(const float* F, float* out) {\n" << centres_body << "    }\n";
     o << "    template <int MODE>\n    static __device__ __forceinline__ bool self_hit(const float* F, bool h, u64 mask, unsigned& work, const Team& T) {\n"
       << self_body(false) << "        return h;\n    }\n";
@@ -718,6 +744,8 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
         std::string R = "Robot" + std::to_string(k), s = std::to_string(k);
         o << "extern \"C\" __global__ void __launch_bounds__(256) mp_fk_" << s
           << "(const float* q, long long n, float* c) { mp_fk_kernel<" << R << ">(q, n, c); }\n";
+        o << "extern \"C\" __global__ void __launch_bounds__(256) mp_poses_" << s
+          << "(const float* q, long long n, float* p) { mp_poses_kernel<" << R << ">(q, n, p); }\n";
         o << "extern \"C\" __global__ void __launch_bounds__(256) mp_check_" << s
           << "(const float* q, long long n, SceneDev sc, unsigned char* v, float* c, unsigned long long* w) { mp_check_kernel<" << R
           << ">(q, n, sc, v, c, w); }\n";

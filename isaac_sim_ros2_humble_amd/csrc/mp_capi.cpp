@@ -549,6 +549,17 @@ int mp_fk_spheres(const mp_robot* robot, const float* d_q, int64_t n, float* d_c
     });
 }
 
+int mp_fk_link_poses(const mp_robot* robot, const float* d_q, int64_t n, float* d_poses, void* stream) {
+    return guard([&] {
+        need_init();
+        if (!robot || n < 0 || (n && (!d_q || !d_poses))) throw Error(MP_E_INVALID, "bad arguments");
+        if (n == 0) return;
+        long long nn = n;
+        void* args[] = {&d_q, &nn, &d_poses};
+        mp::launch(robot->module->fn("mp_poses_0"), (unsigned)((n + 255) / 256), 256, stream_of(stream), args, "fk");
+    });
+}
+
 int mp_check_states(const mp_robot* robot, const mp_scene* scene, const float* d_q, int64_t n, uint8_t* d_verdict,
                     float* d_centres, void* stream) {
     return guard([&] {

@@ -84,6 +84,7 @@ _SIGS = {
     "mp_scene_update": (C.c_int, [P, P, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int]),
     "mp_scene_destroy": (C.c_int, [P]),
     "mp_fk_spheres": (C.c_int, [P, P, C.c_int64, P, P]),
+    "mp_fk_link_poses": (C.c_int, [P, P, C.c_int64, P, P]),
     "mp_check_states": (C.c_int, [P, P, P, C.c_int64, P, P, P]),
     "mp_state_costs": (C.c_int, [P, P, P, C.c_int64, P, P, P]),
     "mp_check_edges": (C.c_int, [P, P, P, P, C.c_int, C.c_float, P, P, P]),
@@ -286,6 +287,16 @@ def fk_spheres(robot, q):
     dc = DeviceBuffer(n * robot.S * 12)
     check(lib().mp_fk_spheres(robot.h, dq.ptr, n, dc.ptr, None), "mp_fk_spheres")
     return dc.to_array(np.float32, (n, robot.S, 3))
+
+
+def fk_link_poses(robot, q):
+    """Pose of every link in the base frame, [n][L][4][4] (row-major, last row 0 0 0 1)."""
+    q = np.ascontiguousarray(q, np.float32).reshape(-1, robot.dof)
+    n = len(q)
+    dq = DeviceBuffer.from_array(q)
+    dp = DeviceBuffer(n * robot.L * 64)
+    check(lib().mp_fk_link_poses(robot.h, dq.ptr, n, dp.ptr, None), "mp_fk_link_poses")
+    return dp.to_array(np.float32, (n, robot.L, 4, 4))
 
 
 def check_states(robot, scene, q, want_centres=False):

@@ -41,6 +41,7 @@ def lib():
         L.orc_sincosf.argtypes = [C.c_float, P, P]
         L.orc_fk64.argtypes = [P, P, P, P]
         L.orc_fk32.argtypes = [P, P, P]
+        L.orc_link_poses32.argtypes = [P, P, P]
         L.orc_check32.argtypes = [P, P, P, C.c_int64, P, P, P, P, C.c_int]
         L.orc_check_centres32.argtypes = [P, P, P, P, C.c_int64, P]
         L.orc_edges32.argtypes = [P, P, P, P, C.c_int, C.c_float, P, P]
@@ -109,6 +110,13 @@ class OracleRobot:
         out = np.zeros((len(q), self.S, 3), np.float32)
         for n in range(len(q)):
             lib().orc_fk32(self.h, _p(q[n]), _p(out[n]))
+        return out
+
+    def link_poses32(self, q):
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, self.dof)
+        out = np.zeros((len(q), self.m.L, 4, 4), np.float32)
+        for n in range(len(q)):
+            lib().orc_link_poses32(self.h, _p(q[n]), _p(out[n]))
         return out
 
     def sample(self, seed, idx0, n):

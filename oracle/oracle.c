@@ -390,8 +390,8 @@ static void rot_of(const orc_robot* r, int i, float s, float c, aff* out) {
     }
 }
 
-void orc_fk32(const orc_robot* r, const float* q, float* centres) {
-    aff T[MAXL];
+/* FP32 link frames T[L] of one configuration (the GPU's composition and operation order). */
+static void fk32_frames(const orc_robot* r, const float* q, aff* T) {
     memset(&T[0], 0, sizeof(aff));
     T[0].R[0] = T[0].R[4] = T[0].R[8] = 1.0f;
     for (int i = 1; i < r->L; i++) {
@@ -419,6 +419,26 @@ void orc_fk32(const orc_robot* r, const float* q, float* centres) {
             T[i] = *P;
         }
     }
+}
+
+/* Link poses [L][16] row-major 4x4 (last row 0 0 0 1): the FK output of
+ * kin_forward_nvrtc_kernel (KernelGenerator.h:222-406) in FP32. */
+void orc_link_poses32(const orc_robot* r, const float* q, float* poses) {
+    aff T[MAXL];
+    fk32_frames(r, q, T);
+    for (int i = 0; i < r->L; i++) {
+        float* P = poses + 16 * i;
+        for (int a = 0; a < 3; a++) {
+            for (int b = 0; b < 3; b++) P[4 * a + b] = T[i].R[3 * a + b];
+            P[4 * a + 3] = T[i].t[a];
+        }
+        P[12] = 0.0f; P[13] = 0.0f; P[14] = 0.0f; P[15] = 1.0f;
+    }
+}
+
+void orc_fk32(const orc_robot* r, const float* q, float* centres) {
+    aff T[MAXL];
+    fk32_frames(r, q, T);
     for (int s = 0; s < r->S; s++) {
         const aff* M = &T[r->sphere_link[s]];
         float px = r->sphere_pos[3 * s], py = r->sphere_pos[3 * s + 1], pz = r->sphere_pos[3 * s + 2];

@@ -63,6 +63,8 @@ void orc_sincosf(float x, float* s, float* c);
 /* FK.  q: active joints [dof].  centres: [S][3].  link_poses (optional): [L][16] row-major. */
 void orc_fk64(const orc_robot* r, const double* q, double* centres, double* link_poses);
 void orc_fk32(const orc_robot* r, const float* q, float* centres);
+/* FP32 link poses [L][16] row-major (last row 0 0 0 1), the frames orc_fk32 places spheres by. */
+void orc_link_poses32(const orc_robot* r, const float* q, float* poses);
 
 /* Check n configurations q[n][dof].  verdict[n] 1 = valid.  cost (optional) [n] total cost.
  * centres (optional) [n][S][3].  canon_flops (optional) [n]: flops of the canonical-order

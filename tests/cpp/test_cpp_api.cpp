@@ -49,7 +49,8 @@ int main(int argc, char** argv) {
     std::vector<std::vector<float>> spos(S);
     std::vector<float> rad, lo, hi, def;
     std::vector<bool> act;
-    std::vector<std::string> names(L, "l");
+    std::vector<std::string> names;
+    for (int i = 0; i < L; i++) names.push_back("l" + std::to_string(i));
     std::vector<std::vector<bool>> en(L, std::vector<bool>(L));
     for (int i = 0; i < L; i++) {
         jt.push_back((int)m["joint_types"][i]);
@@ -113,6 +114,16 @@ int main(int argc, char** argv) {
     std::vector<float> mc;
     space->checkMotions(space->createStatesFromVector({start}), space->createStatesFromVector({goal}), mf, mc);
     std::printf("space_motion %d %.9g\n", (int)mf[0], mc[0]);
+    // forward kinematics through the states (update / link-pose and sphere-centre getters)
+    auto fk_states = std::dynamic_pointer_cast<CUDAMPLib::SingleArmStates>(space->createStatesFromVector({start, goal}));
+    auto last_link = fk_states->getLinkPoseInBaseLinkHost("l" + std::to_string(L - 1));
+    auto centres = fk_states->getSelfCollisionSpheresPosInBaseLinkHost();
+    std::printf("fk_pose");
+    for (auto& T : last_link)
+        for (int r = 0; r < 4; r++)
+            for (int c = 0; c < 4; c++) std::printf(" %.9g", T.matrix()(r, c));
+    std::printf("\nfk_centres %.9g %.9g %.9g %.9g %.9g %.9g\n", centres[0][0][0], centres[0][0][1], centres[0][0][2],
+                centres[1][0][0], centres[1][0][1], centres[1][0][2]);
     auto samples = space->sample(64);
     std::vector<bool> sf;
     space->checkStates(samples, sf);

@@ -53,6 +53,11 @@ def test_cpp_api(tmp_path):
     assert res["space_feasible"] == ["1", "1"]
     assert res["space_motion"][0] == "0" and abs(float(res["space_motion"][1]) - 1.0) < 1e-6
     assert res["space_samples"][0] == "64"
+    # the last link of the point robot sits at the configuration (three prismatic joints)
+    T = np.array([float(x) for x in res["fk_pose"]]).reshape(2, 4, 4)
+    for k, q in enumerate((start, goal)):
+        assert np.allclose(T[k][:3, :3], np.eye(3)) and np.allclose(T[k][:3, 3], q) and np.allclose(T[k][3], [0, 0, 0, 1])
+    assert np.allclose(np.array([float(x) for x in res["fk_centres"]]).reshape(2, 3), [start, goal])
     # RRG (roadmap between the start and goal groups): waypoints joined by motions that pass the
     # same checker at the space's resolution, start and goal at the ends, around the wall.  The
     # roadmap checks each motion from the new state toward its neighbour, so a path edge passes

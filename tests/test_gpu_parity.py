@@ -45,6 +45,20 @@ def test_fk_bit_exact(name, gpu_robots, request, rng):
 
 
 @pytest.mark.parametrize("name", ["fr3", "fetch"])
+def test_link_poses_bit_exact(name, gpu_robots, request, rng):
+    # every link's pose (ragged last block), equal to the FP32 oracle, within 1e-5 of FP64
+    m = request.getfixturevalue(name)
+    orc = request.getfixturevalue("orc_" + name)
+    q = workloads.uniform_configs(m, 1027, rng)
+    g = native.fk_link_poses(gpu_robots[name], q)
+    assert g.shape == (1027, m.L, 4, 4)
+    p32 = orc.link_poses32(q)
+    assert np.array_equal(g, p32), np.abs(g - p32).max()
+    for k in (0, 513, 1026):
+        assert np.abs(g[k].astype(np.float64) - orc.link_poses64(q[k].astype(np.float64))).max() < 1e-5
+
+
+@pytest.mark.parametrize("name", ["fr3", "fetch"])
 def test_check_states_verdicts_identical(name, gpu_robots, request, scenes, rng):
     m = request.getfixturevalue(name)
     orc = request.getfixturevalue("orc_" + name)

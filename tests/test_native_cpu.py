@@ -38,9 +38,13 @@ def test_generated_source_structure(fr3, fetch):
     r1 = src[src.index("struct Robot1"):src.index("mp_load_cfg") if "mp_load_cfg" in src else None]
     hit1 = r1[r1.index("self_hit"):r1.index("self_cost")]
     assert hit1.count("h |=") == 2534
-    # FK evaluates one sin/cos per active joint only
-    fk = r0[r0.index("void fk("):r0.index("write_centres")]
+    # FK evaluates one sin/cos per active joint only, in the sphere frames and the link poses
+    fk = r0[r0.index("void fk("):r0.index("NLINK")]
     assert fk.count("mp_sincosf(") == 7
+    poses = r0[r0.index("void poses("):r0.index("write_centres")]
+    assert poses.count("mp_sincosf(") == 7 and "static constexpr int NLINK = 14;" in r0
+    assert poses.count("] = 1.0f;") >= 14  # the (0, 0, 0, 1) row of every link
+    assert "mp_poses_0" in src and "mp_poses_1" in src
 
 
 def test_generated_checks_split_across_a_team(fr3):

@@ -108,6 +108,33 @@ def test_fk32_within_1e5_of_fk64(name, request, rng):
     assert err < 1e-5, err
 
 
+@pytest.mark.parametrize("name", ["fr3", "fetch"])
+def test_link_poses32_against_fk64_and_sphere_placement(name, request, rng):
+    # within 1e-5 of the FP64 4x4 products; rigid (orthonormal rotations, last row 0 0 0 1);
+    # and the FP32 sphere centres are exactly the FP32 link pose applied to the sphere offsets
+    m = request.getfixturevalue(name)
+    r = request.getfixturevalue("orc_" + name)
+    q = rng.uniform(m.active_lower, m.active_upper, size=(200, m.dof)).astype(np.float32)
+    P32 = r.link_poses32(q)
+    for k in range(0, 200, 20):
+        T64 = r.link_poses64(q[k].astype(np.float64))
+        assert np.abs(P32[k].astype(np.float64) - T64).max() < 1e-5
+    R = P32[:, :, :3, :3].astype(np.float64)
+    assert np.abs(R @ np.swapaxes(R, -1, -2) - np.eye(3)).max() < 1e-5
+    assert np.array_equal(P32[:, :, 3, :], np.broadcast_to(np.float32([0, 0, 0, 1]), P32[:, :, 3, :].shape))
+    c32 = r.fk32(q)
+    M = P32[:, m.sphere_link]                    # [n][S][4][4]
+    p = m.sphere_pos.astype(np.float32)
+    # row-wise fma chain fma(R0, px, fma(R1, py, fma(R2, pz, t))) in float32
+    want = np.empty_like(c32)
+    for a in range(3):
+        acc = np.float32(M[:, :, a, 3])
+        acc = (M[:, :, a, 2].astype(np.float64) * p[:, 2] + acc).astype(np.float32)
+        acc = (M[:, :, a, 1].astype(np.float64) * p[:, 1] + acc).astype(np.float32)
+        want[:, :, a] = (M[:, :, a, 0].astype(np.float64) * p[:, 0] + acc).astype(np.float32)
+    assert np.array_equal(want, c32)
+
+
 # ------------------------------------------------------------------------------------ collision
 def _point_check(scene, pts, radius):
     m = point_robot(radius)
