@@ -221,6 +221,13 @@ class SingleArmSpace : public BaseSpace {
     mp_scene* scene() const { return scene_; }
     // n samples of the space's stream written to device memory d_q[n][dim] (as sample(n))
     void sampleInto(float* d_q, int n);
+    // Space.h sampleConfigurations: the same into a caller's device buffer
+    void sampleConfigurations(float* d_configurations, int num_of_config) { sampleInto(d_configurations, num_of_config); }
+    // The generated HIP module of this robot (FK, sphere placement, self / environment checks
+    // and the planning kernels), compiled with hipRTC for gfx950.  Both reference hooks
+    // (generateFKKernelSourceCode, generateRobotCollisionModelSourceCode) return it.
+    std::string generateFKKernelSourceCode() const;
+    std::string generateRobotCollisionModelSourceCode() const { return generateFKKernelSourceCode(); }
 
   private:
     std::vector<float> lower_bound, upper_bound, defaults_;

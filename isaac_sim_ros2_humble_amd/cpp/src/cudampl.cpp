@@ -248,6 +248,14 @@ std::vector<std::vector<float>> SingleArmSpace::getJointVectorInActiveJointsFrom
     return as_states(states)->getJointStatesHost();
 }
 
+std::string SingleArmSpace::generateFKKernelSourceCode() const {
+    const size_t n = mp_robot_source(robot_, nullptr, 0);
+    std::string src(n + 1, '\0');
+    mp_robot_source(robot_, &src[0], n + 1);
+    src.resize(n);
+    return src;
+}
+
 void SingleArmSpace::sampleInto(float* d_q, int n) {
     if (n < 0) throw std::runtime_error("sample: negative count");
     mpcpp::check(mp_sample(robot_, seed_, sample_offset_, n, d_q, nullptr), "mp_sample");

@@ -124,6 +124,9 @@ int main(int argc, char** argv) {
             for (int c = 0; c < 4; c++) std::printf(" %.9g", T.matrix()(r, c));
     std::printf("\nfk_centres %.9g %.9g %.9g %.9g %.9g %.9g\n", centres[0][0][0], centres[0][0][1], centres[0][0][2],
                 centres[1][0][0], centres[1][0][1], centres[1][0][2]);
+    const std::string gen = space->generateFKKernelSourceCode();
+    std::printf("space_source %d %d\n", (int)(gen.find("mp_plan") != std::string::npos),
+                (int)(gen == space->generateRobotCollisionModelSourceCode()));
     auto samples = space->sample(64);
     std::vector<bool> sf;
     space->checkStates(samples, sf);

@@ -53,6 +53,7 @@ def test_cpp_api(tmp_path):
     assert res["space_feasible"] == ["1", "1"]
     assert res["space_motion"][0] == "0" and abs(float(res["space_motion"][1]) - 1.0) < 1e-6
     assert res["space_samples"][0] == "64"
+    assert res["space_source"] == ["1", "1"]
     # the last link of the point robot sits at the configuration (three prismatic joints)
     T = np.array([float(x) for x in res["fk_pose"]]).reshape(2, 4, 4)
     for k, q in enumerate((start, goal)):
