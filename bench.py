@@ -283,6 +283,7 @@ def main():
     ap.add_argument("--team-waves", type=int, default=0, help="planning launch shape (0: tuned / default)")
     ap.add_argument("--waves", type=int, default=0, help="planning grid in waves (0: tuned / resident)")
     ap.add_argument("--teams-per-query", type=int, default=0, help="teams per query (0: tuned / unbounded)")
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per iteration, 32 or 64 (0: tuned / default)")
     ap.add_argument("--skip", nargs="*", default=[], help="parts to skip: tune plain serve c2 c3 mbm cpu (cpu also skips c1)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -314,10 +315,12 @@ def main():
             best, cands = native.rrtc_autotune(c_rob[:n], c_sc[:n], c_s[:n], c_g[:n], native.default_params(seed=77),
                                                per_query=per_query, reps=2)
             tune[name] = {"team_waves": best.team_waves, "waves": best.waves, "teams_per_query": best.teams_per_query,
-                          "calibration_queries": n, "candidates": cands}
-            log("autotune %s: team_waves %d waves %d teams_per_query %d" % (name, best.team_waves, best.waves,
-                                                                             best.teams_per_query))
-    params = native.default_params(seed=0, team_waves=a.team_waves, waves=a.waves, teams_per_query=a.teams_per_query)
+                          "lanes_per_iteration": best.lanes_per_iteration, "calibration_queries": n,
+                          "candidates": cands}
+            log("autotune %s: team_waves %d waves %d teams_per_query %d lanes_per_iteration %d" % (
+                name, best.team_waves, best.waves, best.teams_per_query, best.lanes_per_iteration))
+    params = native.default_params(seed=0, team_waves=a.team_waves, waves=a.waves, teams_per_query=a.teams_per_query,
+                                   lanes_per_iteration=a.lanes)
 
     # the 256 queries are packed once (host arrays, handles of resident scenes and robots); each
     # step submits them, runs the persistent launch and has every path on the host
@@ -353,8 +356,9 @@ def main():
     # one more (untimed) step with the planner's phase profile on: where the waves spend their time
     native.plan_profile(enable=True)
     params.seed = 999
-    shape = ({"team_waves": a.team_waves, "waves": a.waves, "teams_per_query": a.teams_per_query}
-             if a.team_waves or a.waves or a.teams_per_query else tune.get("batch"))
+    shape = ({"team_waves": a.team_waves, "waves": a.waves, "teams_per_query": a.teams_per_query,
+              "lanes_per_iteration": a.lanes}
+             if a.team_waves or a.waves or a.teams_per_query or a.lanes else tune.get("batch"))
     step(params)
     prof = native.plan_profile(enable=False)
     wave = max(prof["wave"], 1)
@@ -378,7 +382,8 @@ def main():
            "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic("mp_plan"),
                         "kernel": "mp_plan (batched)", "flops_per_launch": float(np.mean(flops)),
-                        "launch_shape": {k: shape[k] for k in ("team_waves", "waves", "teams_per_query")} if shape else None},
+                        "launch_shape": {k: shape[k] for k in ("team_waves", "waves", "teams_per_query",
+                                                               "lanes_per_iteration")} if shape else None},
            "autotune": tune}
 
     if "plain" not in a.skip:

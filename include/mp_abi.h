@@ -199,6 +199,10 @@ typedef struct {
     int teams_per_query;   /* most teams growing one query's trees at once (0 = tuned, or
                               unbounded): more teams on one tree mostly add redundant
                               extensions; surplus teams leave the launch */
+    int lanes_per_iteration; /* 64: one iteration per team pass, chunks of up to 64 states;
+                              32: two iterations per pass, lanes 0-31 and 32-63 (the
+                              reference's 32-thread CPRRTC block, needs max_step <= 32);
+                              0 = tuned default */
 } mp_rrtc_params;
 
 /* Defaults: resolution 0.02, max_step 32, max_iterations 10000, tree_capacity 65536,
@@ -238,17 +242,19 @@ int mp_rrtc_solve_batch(int nq, mp_robot* const* robots, const mp_scene* const* 
 
 /* Autotuner of the planning launch shape (no counterpart in the reference; CPRRTC fixes one
  * block of threads per worker).  Solves the sample queries once per candidate shape and seed
- * (team_waves 1, 2, 4 x total waves x teams per query 0, 16, 32, 64, 128; per_query = 1 tunes the
- * one-query-per-launch shape by solving the sample one query at a time, 0 tunes the batch shape
- * with the sample as one batch), scores each by its mean planning-kernel time, and returns the
- * fastest in *best (params with team_waves, waves and teams_per_query set).  The choice is kept
- * for this robot set and class, in memory and in the code-object cache directory, and used by
- * every later solve or server whose params leave team_waves, waves and teams_per_query at 0.
- * results (may be NULL) receives up to cap candidates. */
+ * (team_waves 1, 2, 4 x total waves x teams per query 0, 16, 32, 64, 128 x lanes per iteration
+ * 64, 32 (32 when max_step <= 32); per_query = 1 tunes the one-query-per-launch shape by solving
+ * the sample one query at a time, 0 tunes the batch shape with the sample as one batch), scores
+ * each by its mean planning-kernel time, and returns the fastest in *best (params with
+ * team_waves, waves, teams_per_query and lanes_per_iteration set).  The choice is kept for this
+ * robot set and class, in memory and in the code-object cache directory, and used by every later
+ * solve or server whose params leave those four fields at 0.  results (may be NULL) receives up
+ * to cap candidates. */
 typedef struct {
     int team_waves;
     int waves;             /* 0 = every team the device keeps resident */
     int teams_per_query;   /* 0 = unbounded */
+    int lanes_per_iteration;
     double kernel_ms;      /* mean planning-kernel time over the seeds (summed over queries
                               when per_query) */
 } mp_tune_result;

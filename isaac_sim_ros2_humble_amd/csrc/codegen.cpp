@@ -767,11 +767,11 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "    Q.tree[0] = mp_sld(&p->tree[0]); Q.tree[1] = mp_sld(&p->tree[1]);\n"
          "    Q.parent[0] = mp_sld(&p->parent[0]); Q.parent[1] = mp_sld(&p->parent[1]); Q.path = mp_sld(&p->path);\n"
          "    Q.seed = mp_sld(&p->seed); Q.robot = mp_sld(&p->robot); Q.dof = mp_sld(&p->dof); Q.cap = mp_sld(&p->cap);\n"
-         "    Q.max_iters = mp_sld(&p->max_iters); Q.max_path = mp_sld(&p->max_path);\n"
+         "    Q.max_iters = mp_sld(&p->max_iters); Q.max_path = mp_sld(&p->max_path); Q.dof_max = mp_sld(&p->dof_max);\n"
          "}\n";
     // planning kernels: one team (workgroup of W waves, W = blockDim.x / 64) per query iteration
     const std::string team_setup =
-        "    __shared__ u64 team_red[MP_TEAM_MAX];\n"
+        "    __shared__ u64 team_red[2 * MP_TEAM_MAX];\n"
         "    __shared__ int team_flag;\n"
         "    Team T;\n"
         "    T.w = mp_uniform((int)(threadIdx.x >> 6));\n"
@@ -785,6 +785,7 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "    const int budget = mp_uniform(ctl.ctl[3]);\n"
          "    const int per_ticket = mp_uniform(ctl.ctl[7]) > 0 ? mp_uniform(ctl.ctl[7]) : 1;\n"
          "    const int cap = mp_uniform(ctl.ctl[8]);\n"
+         "    const bool pair = mp_uniform(ctl.ctl[9]) != 0;  // two iterations per pass (32 lanes each)\n"
          "    if (n_active <= 0) return;\n"
       << team_setup <<
          "    const bool lead = T.w == 0;\n"
@@ -805,7 +806,8 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "            bool more = false;\n"
          "            switch (Q.robot) {\n";
     for (size_t k = 0; k < robots.size(); k++)
-        o << "                case " << k << ": more = mp_plan_iteration<Robot" << k << ">(ctl, Q, st, pl, T); break;\n";
+        o << "                case " << k << ": more = pair ? mp_plan_pair<Robot" << k << ">(ctl, Q, st, pl, T) : mp_plan_iteration<Robot"
+          << k << ">(ctl, Q, st, pl, T); break;\n";
     o << "                default: break;\n            }\n            if (!more) break;\n        }\n"
          "        mp_prof_add(pl, MP_PROF_WAVE, mp_now(pl) - t_wave);\n"
          "        mp_prof_flush(ctl, pl);\n"
@@ -864,7 +866,8 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "            bool more = false;\n"
          "            switch (Q.robot) {\n";
     for (size_t k = 0; k < robots.size(); k++)
-        o << "                case " << k << ": more = mp_plan_iteration<Robot" << k << ">(ctl, Q, st, pl, T); break;\n";
+        o << "                case " << k << ": more = pair ? mp_plan_pair<Robot" << k << ">(ctl, Q, st, pl, T) : mp_plan_iteration<Robot"
+          << k << ">(ctl, Q, st, pl, T); break;\n";
     o << "                default: break;\n            }\n            if (!more) break;\n        }\n"
          "        if (cap > 0 && lead && lane == 0) __hip_atomic_fetch_add(&st->workers, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
          "    }\n"

@@ -98,7 +98,7 @@ size_t slot_bytes(int cap, int dof_max);  // device bytes of one slot
 // Runs f, converting exceptions to an ABI status code (message in mp_last_error()).
 int guard_call(const std::function<void()>& f);
 void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params& p, int team, int teams,
-                       int cap, hipStream_t s);
+                       int cap, int lanes, hipStream_t s);
 
 // Launch shape of the planning kernel: waves per team, total waves (0: resident teams) and
 // teams per query (0: unbounded).
@@ -106,6 +106,7 @@ struct Shape {
     int team = 0;
     int waves = 0;
     int cap = 0;
+    int lanes = 0;  // lanes per iteration (64 or 32; 0 = default)
 };
 std::string robot_set_key(const std::vector<mp_robot*>& robots);
 Shape tuned_shape(const std::string& set_key, int cls);  // team 0 when nothing is tuned
@@ -115,6 +116,9 @@ void harvest(const SlotPool& pool, int slot, int dof, const QueryStateHost& st, 
 constexpr int kDefaultTeamWaves = 4;
 // Iterations a team runs on the query its ticket picked (solve paths; the server uses 1).
 constexpr int kItersPerTicket = 4;
+constexpr int kDefaultLanes = 64;  // lanes per iteration when neither given nor tuned
+// control-block flag of the paired iteration: lanes per iteration 32 (max_step <= 32 checked)
+inline int pair_flag(int lanes) { return lanes == 32 ? 1 : 0; }
 // Resident teams of `team_waves` waves the planning kernel can keep on the device.
 int default_teams(hipFunction_t plan_fn, int team_waves);
 int team_size(const mp_rrtc_params& p);  // waves per team (the default when p.team_waves is 0)

@@ -23,10 +23,8 @@ SlotPool::SlotPool(int ns, int c, int d, int mp_) : nslots(ns), cap(c), dof_max(
     try {
         MP_HIP(hipMalloc(&trees, (size_t)nslots * 2 * dof_max * cap * sizeof(float)));
         MP_HIP(hipMalloc(&parents, (size_t)nslots * 2 * cap * sizeof(int)));
-        // non-coherent pinned memory: host reads are cached (fine-grained host memory reads
-        // uncached, which made harvesting a large batch cost milliseconds)
         MP_HIP(hipHostMalloc(&hpaths, (size_t)nslots * max_path * dof_max * sizeof(float),
-                             hipHostMallocMapped | hipHostMallocNonCoherent));
+                             hipHostMallocMapped | hipHostMallocCoherent));
         MP_HIP(hipHostGetDevicePointer((void**)&paths, hpaths, 0));
         MP_HIP(hipMalloc(&dq, (size_t)nslots * sizeof(QueryDevHost)));
         MP_HIP(hipMalloc(&dst, (size_t)nslots * sizeof(QueryStateHost)));
@@ -150,6 +148,10 @@ void check_params(const mp_rrtc_params* p) {
     if (p->team_waves != 0 && p->team_waves != 1 && p->team_waves != 2 && p->team_waves != 4)
         throw Error(MP_E_INVALID, "team_waves must be 0, 1, 2 or 4");
     if (p->teams_per_query < 0) throw Error(MP_E_INVALID, "teams_per_query must be >= 0");
+    if (p->lanes_per_iteration != 0 && p->lanes_per_iteration != 32 && p->lanes_per_iteration != 64)
+        throw Error(MP_E_INVALID, "lanes_per_iteration must be 0, 32 or 64");
+    if (p->lanes_per_iteration == 32 && p->max_step > 32)
+        throw Error(MP_E_INVALID, "lanes_per_iteration 32 needs max_step <= 32");
 }
 
 double now_ms() {
@@ -166,7 +168,7 @@ void plan_release_workspaces() {
 // Runs the queries already described in pool.hq[0..nq) to completion in one persistent launch
 // of `teams` workgroups of `team` waves.
 void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params& p, int team, int teams,
-                       int cap, hipStream_t s) {
+                       int cap, int lanes, hipStream_t s) {
     const unsigned block = 64u * (unsigned)team;
     MP_HIP(hipMemcpyAsync(pool.dq, pool.hq, (size_t)nq * sizeof(QueryDevHost), hipMemcpyHostToDevice, s));
     int* h = pool.hpinned;
@@ -184,6 +186,7 @@ void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params
     }();
     h[7] = per_ticket;
     h[8] = cap;
+    h[9] = pair_flag(lanes);
     for (int k = 0; k < nq; k++) h[16 + k] = k;
     MP_HIP(hipMemcpyAsync(pool.dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, s));
     MP_HIP(hipMemcpyAsync(pool.dactive, h + 16, (size_t)nq * sizeof(int), hipMemcpyHostToDevice, s));
@@ -223,8 +226,9 @@ void harvest(const SlotPool& pool, int slot, int dof, const QueryStateHost& st, 
 namespace mp {
 
 // ------------------------------------------------------------------------------ launch shapes
-// The autotuner's choice of team size and grid per robot set and class (0: one query per
-// launch, 1: batches), kept in memory and in <cache dir>/tune_<hash>.txt ("class team waves").
+// The autotuner's choice of launch shape per robot set and class (0: one query per launch,
+// 1: batches), kept in memory and in <cache dir>/tune_<hash>.txt, one line per class:
+// "class team waves teams_per_query lanes_per_iteration".
 namespace {
 std::mutex g_tune_mu;
 std::map<std::string, std::pair<Shape, bool>> g_tuned;  // key#class -> (shape, looked up)
@@ -257,9 +261,13 @@ Shape tuned_shape(const std::string& set_key, int cls) {
     if (it != g_tuned.end()) return it->second.first;
     Shape sh;
     if (FILE* f = std::fopen(tune_file(set_key).c_str(), "r")) {
-        int c, t, w, q;
-        while (std::fscanf(f, "%d %d %d %d", &c, &t, &w, &q) == 4)
-            if (c == cls && (t == 1 || t == 2 || t == 4) && w >= 0 && q >= 0) sh = Shape{t, w, q};
+        char line[128];
+        while (std::fgets(line, sizeof(line), f)) {
+            int c, t, w, q, l = 0;
+            const int n = std::sscanf(line, "%d %d %d %d %d", &c, &t, &w, &q, &l);
+            if (n >= 4 && c == cls && (t == 1 || t == 2 || t == 4) && w >= 0 && q >= 0 && (l == 0 || l == 32 || l == 64))
+                sh = Shape{t, w, q, l};
+        }
         std::fclose(f);
     }
     g_tuned[k] = {sh, true};
@@ -275,7 +283,8 @@ void store_shape(const std::string& set_key, int cls, Shape sh) {
         auto it = g_tuned.find(set_key + "@" + std::to_string(c));
         if (it != g_tuned.end() && it->second.first.team > 0)
             lines += std::to_string(c) + " " + std::to_string(it->second.first.team) + " " +
-                     std::to_string(it->second.first.waves) + " " + std::to_string(it->second.first.cap) + "\n";
+                     std::to_string(it->second.first.waves) + " " + std::to_string(it->second.first.cap) + " " +
+                     std::to_string(it->second.first.lanes) + "\n";
     }
     const std::string path = tune_file(set_key), tmp = path + ".tmp";
     if (FILE* f = std::fopen(tmp.c_str(), "w")) {
@@ -328,17 +337,21 @@ void solve_impl(const char* tag, int nq, mp_robot* const* robots, const mp_scene
     int W = params->team_waves > 0 ? params->team_waves : kDefaultTeamWaves;
     int waves = params->waves > 0 ? params->waves : default_wave_count;
     int cap = params->teams_per_query;
-    if (params->team_waves == 0 && params->waves == 0 && params->teams_per_query == 0) {
+    int lanes = params->lanes_per_iteration;
+    if (params->team_waves == 0 && params->waves == 0 && params->teams_per_query == 0 &&
+        params->lanes_per_iteration == 0) {
         const Shape tuned = tuned_shape(robot_set_key(uniq), nq == 1 ? 0 : 1);
         if (tuned.team > 0) {
             W = tuned.team;
             waves = tuned.waves;
             cap = tuned.cap;
+            lanes = tuned.lanes;
         }
     }
+    if (lanes == 0 || params->max_step > 32) lanes = params->max_step > 32 ? 64 : kDefaultLanes;
     int teams = waves > 0 ? std::max(1, waves / W) : default_teams(mod->fn("mp_plan"), W);
     if (params->exclusive) teams = nq;
-    run_to_completion(*mod, pool, nq, *params, W, teams, cap, s);
+    run_to_completion(*mod, pool, nq, *params, W, teams, cap, lanes, s);
     for (int k = 0; k < nq; k++) harvest(pool, k, robots[k]->model.dof, pool.hst_pinned[k], &out[k]);
     double lat = now_ms() - t0;
     for (int k = 0; k < nq; k++) out[k].latency_ms = lat;
@@ -360,6 +373,7 @@ void mp_rrtc_default_params(mp_rrtc_params* p) {
     p->exclusive = 0;
     p->team_waves = 0;
     p->teams_per_query = 0;
+    p->lanes_per_iteration = 0;
 }
 
 int mp_rrtc_solve(mp_robot* robot, const mp_scene* scene, const float* start, const float* goal,
@@ -393,26 +407,30 @@ int mp_rrtc_autotune(int nq, mp_robot* const* robots, const mp_scene* const* sce
         // candidates: waves per team x total waves (0 = every team the device keeps resident) x
         // teams per query (0 = unbounded; single queries only use the grid)
         struct Cand {
-            int team, waves, cap;
+            int team, waves, cap, lanes;
         };
         std::vector<Cand> cands;
-        for (int t : {1, 2, 4}) {
-            if (per_query) {
-                for (int w : {256, 512, 1024, 2048}) cands.push_back({t, w, 0});
-            } else {
-                for (int w : {0, 1024})
-                    for (int c : {0, 16, 32, 64, 128}) cands.push_back({t, w, c});
+        std::vector<int> lane_opts{64};
+        if (params->max_step <= 32) lane_opts.push_back(32);
+        for (int l : lane_opts)
+            for (int t : {1, 2, 4}) {
+                if (per_query) {
+                    for (int w : {256, 512, 1024, 2048}) cands.push_back({t, w, 0, l});
+                } else {
+                    for (int w : {0, 1024})
+                        for (int c : {0, 16, 32, 64, 128}) cands.push_back({t, w, c, l});
+                }
             }
-        }
         std::vector<mp_path> out((size_t)nq);
         int n = 0;
-        Cand best_c{0, 0, 0};
+        Cand best_c{0, 0, 0, 0};
         double bms = 1e300;
         for (const Cand& cd : cands) {
             mp_rrtc_params p = *params;
             p.team_waves = cd.team;
             p.waves = cd.waves;
             p.teams_per_query = cd.cap;
+            p.lanes_per_iteration = cd.lanes;
             double ms = 0.0;
             for (int r = 0; r < reps; r++) {
                 p.seed = params->seed + 7919ull * (uint64_t)r;
@@ -429,7 +447,7 @@ int mp_rrtc_autotune(int nq, mp_robot* const* robots, const mp_scene* const* sce
                 }
             }
             ms /= reps;
-            if (results && n < cap) results[n] = mp_tune_result{cd.team, cd.waves, cd.cap, ms};
+            if (results && n < cap) results[n] = mp_tune_result{cd.team, cd.waves, cd.cap, cd.lanes, ms};
             n++;
             if (ms < bms) {
                 bms = ms;
@@ -437,11 +455,12 @@ int mp_rrtc_autotune(int nq, mp_robot* const* robots, const mp_scene* const* sce
             }
         }
         if (n_results) *n_results = n;
-        mp::store_shape(key, cls, mp::Shape{best_c.team, best_c.waves, best_c.cap});
+        mp::store_shape(key, cls, mp::Shape{best_c.team, best_c.waves, best_c.cap, best_c.lanes});
         *best = *params;
         best->team_waves = best_c.team;
         best->waves = best_c.waves;
         best->teams_per_query = best_c.cap;
+        best->lanes_per_iteration = best_c.lanes;
     });
 }
 

@@ -56,7 +56,7 @@ struct mp_server {
     std::vector<int> robot_slot;  // robot index -> the module's robot slot
     std::shared_ptr<mp::Module> mod;
     std::unique_ptr<mp::SlotPool> pool;
-    int nslots = 0, teams = 0, team = 1, cap = 0, budget = 0;
+    int nslots = 0, teams = 0, team = 1, cap = 0, lanes = 64, budget = 0;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
@@ -93,6 +93,7 @@ void mp_server::step(const std::vector<int>& list, int n_new) {
     h[6] = prm.params.exclusive ? 1 : 0;
     h[7] = 1;  // one iteration per ticket: the step's budget stays a budget of iterations
     h[8] = cap;
+    h[9] = mp::pair_flag(lanes);
     for (size_t k = 0; k < list.size(); k++) h[16 + k] = list[k];
     for (int k = (int)list.size() - n_new; k < (int)list.size(); k++) {
         int slot = list[k];
@@ -236,17 +237,25 @@ int mp_server_create(const mp_server_params* params, mp_server** out) {
         if (p.team_waves != 0 && p.team_waves != 1 && p.team_waves != 2 && p.team_waves != 4)
             throw Error(MP_E_INVALID, "team_waves must be 0, 1, 2 or 4");
         if (p.teams_per_query < 0) throw Error(MP_E_INVALID, "teams_per_query must be >= 0");
+        if (p.lanes_per_iteration != 0 && p.lanes_per_iteration != 32 && p.lanes_per_iteration != 64)
+            throw Error(MP_E_INVALID, "lanes_per_iteration must be 0, 32 or 64");
+        if (p.lanes_per_iteration == 32 && p.max_step > 32)
+            throw Error(MP_E_INVALID, "lanes_per_iteration 32 needs max_step <= 32");
         s->team = mp::team_size(p);
         s->cap = p.teams_per_query;
+        s->lanes = p.lanes_per_iteration;
         int waves = params->waves;
-        if (p.team_waves == 0 && params->waves == 0 && p.teams_per_query == 0) {  // the tuned batch shape
+        if (p.team_waves == 0 && params->waves == 0 && p.teams_per_query == 0 &&
+            p.lanes_per_iteration == 0) {  // the tuned batch shape
             const mp::Shape tuned = mp::tuned_shape(mp::robot_set_key(s->robots), 1);
             if (tuned.team > 0) {
                 s->team = tuned.team;
                 waves = tuned.waves;
                 s->cap = tuned.cap;
+                s->lanes = tuned.lanes;
             }
         }
+        if (s->lanes == 0 || p.max_step > 32) s->lanes = p.max_step > 32 ? 64 : mp::kDefaultLanes;
         s->teams = waves > 0 ? std::max(1, waves / s->team) : mp::default_teams(s->mod->fn("mp_plan"), s->team);
         if (p.exclusive) s->teams = s->nslots;
         s->budget = params->step_iterations > 0 ? params->step_iterations : 2 * s->teams;

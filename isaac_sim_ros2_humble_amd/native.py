@@ -30,7 +30,8 @@ class RobotDesc(C.Structure):
 class RRTCParams(C.Structure):
     _fields_ = [("resolution", C.c_float), ("max_step", C.c_int), ("max_iterations", C.c_int),
                 ("tree_capacity", C.c_int), ("waves", C.c_int), ("max_path", C.c_int), ("seed", C.c_uint64),
-                ("exclusive", C.c_int), ("team_waves", C.c_int), ("teams_per_query", C.c_int)]
+                ("exclusive", C.c_int), ("team_waves", C.c_int), ("teams_per_query", C.c_int),
+                ("lanes_per_iteration", C.c_int)]
 
 
 class Path(C.Structure):
@@ -41,7 +42,8 @@ class Path(C.Structure):
 
 
 class TuneResult(C.Structure):
-    _fields_ = [("team_waves", C.c_int), ("waves", C.c_int), ("teams_per_query", C.c_int), ("kernel_ms", C.c_double)]
+    _fields_ = [("team_waves", C.c_int), ("waves", C.c_int), ("teams_per_query", C.c_int),
+                ("lanes_per_iteration", C.c_int), ("kernel_ms", C.c_double)]
 
 
 class ServerParams(C.Structure):
@@ -490,7 +492,7 @@ def rrtc_solve_batch(robots, scenes, starts, goals, params=None):
 
 def rrtc_autotune(robots, scenes, starts, goals, params=None, per_query=False, reps=2):
     """Tunes the planning launch shape on the sample (mp_rrtc_autotune); returns the best params
-    and the candidates as dicts (team_waves, waves, kernel_ms)."""
+    and the candidates as dicts (team_waves, waves, teams_per_query, lanes_per_iteration, kernel_ms)."""
     params = params or default_params()
     nq = len(robots)
     s16 = np.zeros((nq, 16), np.float32)
@@ -500,14 +502,15 @@ def rrtc_autotune(robots, scenes, starts, goals, params=None, per_query=False, r
         g16[k, :robots[k].dof] = goals[k]
     rh = (C.c_void_p * nq)(*[r.h for r in robots])
     sh = (C.c_void_p * nq)(*[s.h for s in scenes])
-    res = (TuneResult * 64)()
+    res = (TuneResult * 128)()
     n = C.c_int()
     best = RRTCParams()
     check(lib().mp_rrtc_autotune(nq, C.cast(rh, C.c_void_p), C.cast(sh, C.c_void_p), _p(s16), _p(g16), C.byref(params),
-                                 1 if per_query else 0, reps, C.cast(res, C.c_void_p), 64, C.byref(n), C.byref(best)),
+                                 1 if per_query else 0, reps, C.cast(res, C.c_void_p), 128, C.byref(n), C.byref(best)),
           "mp_rrtc_autotune")
     return best, [{"team_waves": r.team_waves, "waves": r.waves, "teams_per_query": r.teams_per_query,
-                   "kernel_ms": r.kernel_ms} for r in res[:min(n.value, 64)]]
+                   "lanes_per_iteration": r.lanes_per_iteration, "kernel_ms": r.kernel_ms}
+                  for r in res[:min(n.value, 128)]]
 
 
 class Server:

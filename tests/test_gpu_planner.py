@@ -88,7 +88,8 @@ def test_invalid_start_reported(fr3):
     assert res["status"] == "invalid" and len(res["path"]) == 0
 
 
-def test_exhausted_and_tree_full():
+@pytest.mark.parametrize("lanes", [64, 32])
+def test_exhausted_and_tree_full(lanes):
     m = point_robot(0.02, lim=1.0)
     r = _robot(m)
     # goal enclosed by a box shell: unreachable
@@ -97,13 +98,17 @@ def test_exhausted_and_tree_full():
         (0, 0.15, 0, [0.2, 0.01, 0.2]), (0, 0, -0.15, [0.2, 0.2, 0.01]), (0, 0, 0.15, [0.2, 0.2, 0.01])]]
     scene = native.Scene(Scene(boxes=np.array(walls)))
     s, g = np.array([-0.5, 0, 0], np.float32), np.array([0.5, 0, 0], np.float32)
-    res = native.rrtc_solve(r, scene, s, g, native.default_params(max_iterations=300, seed=2))
-    assert res["status"] == "exhausted" and res["iterations"] == 300
-    res = native.rrtc_solve(r, scene, s, g, native.default_params(max_iterations=100000, tree_capacity=64, seed=2))
+    for budget in (300, 301):  # an odd budget leaves the paired iteration's second half idle
+        res = native.rrtc_solve(r, scene, s, g, native.default_params(max_iterations=budget, seed=2,
+                                                                      lanes_per_iteration=lanes))
+        assert res["status"] == "exhausted" and res["iterations"] == budget
+    res = native.rrtc_solve(r, scene, s, g, native.default_params(max_iterations=100000, tree_capacity=64, seed=2,
+                                                                  lanes_per_iteration=lanes))
     assert res["status"] == "tree_full"
 
 
-def test_exclusive_mode_deterministic_and_batch_equals_solo(fr3, fetch):
+@pytest.mark.parametrize("lanes", [64, 32])
+def test_exclusive_mode_deterministic_and_batch_equals_solo(fr3, fetch, lanes):
     rng = np.random.default_rng(5)
     queries = []
     for k in range(6):
@@ -112,11 +117,12 @@ def test_exclusive_mode_deterministic_and_batch_equals_solo(fr3, fetch):
         pair = workloads.sample_pair(m, scene, rng, gpu_check, gpu_edge, want_free_line=(k % 3 == 0))
         assert pair is not None
         queries.append((m, scene, pair[0], pair[1]))
-    p = native.default_params(exclusive=1, seed=11)
-    solo = [native.rrtc_solve(_robot(m), native.Scene(sc), s, g, native.default_params(exclusive=1, seed=11 + k))
+    p = native.default_params(exclusive=1, seed=11, lanes_per_iteration=lanes)
+    solo = [native.rrtc_solve(_robot(m), native.Scene(sc), s, g,
+                              native.default_params(exclusive=1, seed=11 + k, lanes_per_iteration=lanes))
             for k, (m, sc, s, g) in enumerate(queries)]
     again = native.rrtc_solve(_robot(queries[0][0]), native.Scene(queries[0][1]), queries[0][2], queries[0][3],
-                              native.default_params(exclusive=1, seed=11))
+                              native.default_params(exclusive=1, seed=11, lanes_per_iteration=lanes))
     assert np.array_equal(again["path"], solo[0]["path"])
     batch = native.rrtc_solve_batch([_robot(m) for m, _, _, _ in queries], [native.Scene(sc) for _, sc, _, _ in queries],
                                     [s for _, _, s, _ in queries], [g for _, _, _, g in queries], p)
@@ -127,7 +133,8 @@ def test_exclusive_mode_deterministic_and_batch_equals_solo(fr3, fetch):
             _assert_valid(m, sc, solo[k], s, g)
 
 
-def test_exclusive_mode_team_size_invariant(fr3, fetch):
+@pytest.mark.parametrize("lanes", [64, 32])
+def test_exclusive_mode_team_size_invariant(fr3, fetch, lanes):
     """One team per query: the nearest-neighbour ties go to the lowest index and a check's
     verdict on its deciding lanes is exact for any split of the work across the team's waves,
     so the planner's decisions, and the paths, are the same for 1, 2 and 4 waves per team."""
@@ -145,7 +152,8 @@ def test_exclusive_mode_team_size_invariant(fr3, fetch):
     ref = None
     for team in (1, 2, 4):
         res = native.rrtc_solve_batch(robots, scenes, starts, goals,
-                                      native.default_params(exclusive=1, seed=5, team_waves=team))
+                                      native.default_params(exclusive=1, seed=5, team_waves=team,
+                                                            lanes_per_iteration=lanes))
         got = [(r["status"], r["path"], r["iterations"], r["start_tree"], r["goal_tree"], r["configs_checked"])
                for r in res]
         if ref is None:
@@ -156,7 +164,8 @@ def test_exclusive_mode_team_size_invariant(fr3, fetch):
             assert a[0] == b[0] and np.array_equal(a[1], b[1]) and a[2:] == b[2:], (team, a[0], b[0], a[2:], b[2:])
 
 
-def test_batch_shared_mixed_robots(fr3, fetch):
+@pytest.mark.parametrize("lanes", [64, 32])
+def test_batch_shared_mixed_robots(fr3, fetch, lanes):
     rng = np.random.default_rng(9)
     robots, scenes, starts, goals, models = [], [], [], [], []
     for k in range(24):
@@ -167,7 +176,7 @@ def test_batch_shared_mixed_robots(fr3, fetch):
             continue
         robots.append(_robot(m)); scenes.append(native.Scene(scene)); starts.append(pair[0]); goals.append(pair[1])
         models.append((m, scene))
-    res = native.rrtc_solve_batch(robots, scenes, starts, goals, native.default_params(seed=3))
+    res = native.rrtc_solve_batch(robots, scenes, starts, goals, native.default_params(seed=3, lanes_per_iteration=lanes))
     solved = [r for r in res if r["status"] == "solved"]
     assert len(solved) >= 0.8 * len(res)
     valid = 0
@@ -175,6 +184,35 @@ def test_batch_shared_mixed_robots(fr3, fetch):
         if r["status"] == "solved":
             valid += _assert_valid(m, sc, r, s, g)
     assert valid >= 0.9 * len(solved)
+
+
+@pytest.mark.parametrize("lanes", [64, 32])
+def test_reused_slots_hold_no_earlier_nodes(fr3, fetch, lanes):
+    """Admission empties every coordinate row of a reused slot: batches of the same size run in
+    the same slot pool, here with the robots swapped between slots from one batch to the next
+    and many teams per query, and every solved path keeps the planner's contract."""
+    rng = np.random.default_rng(13)
+    sets = []
+    for swap in (0, 1):
+        qs = []
+        for k in range(16):
+            m = fr3 if (k + swap) % 2 == 0 else fetch
+            scene = workloads.random_scene(rng, workloads.log_uniform_count(rng, 5, 60), *workloads.workspace(m))
+            pair = workloads.sample_pair(m, scene, rng, gpu_check, gpu_edge, want_free_line=False)
+            if pair is not None:
+                qs.append((m, scene, pair[0], pair[1]))
+        sets.append(qs[:12])
+    solved = 0
+    for rep in range(6):
+        qs = sets[rep % 2]
+        res = native.rrtc_solve_batch([_robot(m) for m, _, _, _ in qs], [native.Scene(sc) for _, sc, _, _ in qs],
+                                      [s for _, _, s, _ in qs], [g for _, _, _, g in qs],
+                                      native.default_params(seed=40 + rep, lanes_per_iteration=lanes))
+        for (m, sc, s, g), r in zip(qs, res):
+            if r["status"] == "solved":
+                _assert_valid(m, sc, r, s, g)
+                solved += 1
+    assert solved >= 0.8 * 6 * 12
 
 
 def test_autotune_picks_fastest_candidate_and_is_used(fr3, fetch):
@@ -188,12 +226,13 @@ def test_autotune_picks_fastest_candidate_and_is_used(fr3, fetch):
             continue
         robots.append(_robot(m)); scenes.append(native.Scene(scene)); starts.append(pair[0]); goals.append(pair[1])
     best, cands = native.rrtc_autotune(robots, scenes, starts, goals, native.default_params(seed=4), reps=1)
-    assert len(cands) == 30
+    assert len(cands) == 60
     assert {c["team_waves"] for c in cands} == {1, 2, 4}
     assert {c["teams_per_query"] for c in cands} == {0, 16, 32, 64, 128}
+    assert {c["lanes_per_iteration"] for c in cands} == {32, 64}
     fastest = min(cands, key=lambda c: c["kernel_ms"])
-    assert (best.team_waves, best.waves, best.teams_per_query) == (
-        fastest["team_waves"], fastest["waves"], fastest["teams_per_query"])
+    assert (best.team_waves, best.waves, best.teams_per_query, best.lanes_per_iteration) == (
+        fastest["team_waves"], fastest["waves"], fastest["teams_per_query"], fastest["lanes_per_iteration"])
     assert all(c["kernel_ms"] > 0 for c in cands)
     # later batches with team_waves = waves = 0 run with the stored shape
     res = native.rrtc_solve_batch(robots, scenes, starts, goals, native.default_params(seed=9))

@@ -47,6 +47,17 @@ def test_generated_source_structure(fr3, fetch):
     assert "mp_poses_0" in src and "mp_poses_1" in src
 
 
+def test_query_descriptor_loads_every_field_the_kernels_read(fr3):
+    # the planning kernels read these QueryDev fields after mp_load_query (admission empties a
+    # reused slot over dof_max coordinate rows)
+    src = native.generate_source([fr3])
+    body = src[src.index("mp_load_query("):]
+    body = body[:body.index("\n}\n")]
+    for f in ("tree[0]", "tree[1]", "parent[0]", "parent[1]", "path", "seed", "robot", "dof", "cap", "max_iters",
+              "max_path", "dof_max", "scene.box", "scene.sph", "scene.cyl", "scene.cap"):
+        assert "Q.%s = mp_sld(&p->%s)" % (f, f) in body, f
+
+
 def test_generated_checks_split_across_a_team(fr3):
     """Every self-collision block is guarded by its team share and every obstacle loop strides by
     the team size, so a team of W waves covers each block and obstacle exactly once."""
