@@ -822,6 +822,9 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "            if (lane == 0) t = __hip_atomic_fetch_add((u32*)ctl.ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
          "            t = (u32)__shfl((int)t, 0);\n"
          "            const int running = mp_uniform(__hip_atomic_load(ctl.ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));\n"
+         "            // the cap grows as queries finish, so the last ones share every team of the launch\n"
+         "            const int fair = running > 0 ? (int)((gridDim.x + (unsigned)running - 1u) / (unsigned)running) : 1;\n"
+         "            const int capq = cap > 0 ? (cap > fair ? cap : fair) : 0;\n"
          "            if (!(budget > 0 && t >= (u32)budget) && running > 0) {\n"
          "                qi = -1;\n"
          "                bool full = false;  // a running query was passed over: it had its teams\n"
@@ -839,11 +842,11 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "                    u64 b = __ballot(run);\n"
          "                    while (b) {\n"
          "                        const int cand = __shfl(a, __builtin_ctzll(b));\n"
-         "                        if (cap <= 0) { qi = cand; break; }\n"
+         "                        if (capq <= 0) { qi = cand; break; }\n"
          "                        int old = 0;\n"
          "                        if (lane == 0) old = __hip_atomic_fetch_add(&ctl.st[cand].workers, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
          "                        old = __shfl(old, 0);\n"
-         "                        if (old < cap) { qi = cand; break; }\n"
+         "                        if (old < capq) { qi = cand; break; }\n"
          "                        if (lane == 0) __hip_atomic_fetch_add(&ctl.st[cand].workers, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
          "                        full = true;\n"
          "                        b &= b - 1ull;\n"
