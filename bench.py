@@ -271,8 +271,8 @@ def serve(robots, rob_names, queries, rate, seconds, exclusive=False):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--queries", type=int, default=256)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--rate", type=float, nargs="*", default=[100.0, 1000.0, 5000.0])
