@@ -38,7 +38,9 @@ def planner_states(path, resolution):
     between the trees (shorter than 2 res; only its endpoints are checked, CPRRTC semantics)."""
     out = [path[:1].astype(np.float64)]
     for a, b in zip(path[:-1].astype(np.float64), path[1:].astype(np.float64)):
-        n = max(int(round(np.linalg.norm(b - a) / resolution)), 1)
+        # tree edges are whole multiples of res (up to float32 rounding, < 1e-5 relative); the
+        # join is shorter than 2 res and gets its endpoint only
+        n = max(int(np.floor(np.linalg.norm(b - a) / resolution + 1e-4)), 1)
         out.append(a + (b - a) * (np.arange(1, n + 1)[:, None] / n))
     return np.concatenate(out).astype(np.float32)
 
