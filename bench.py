@@ -279,7 +279,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--c3-seeds", type=int, default=1000)
     ap.add_argument("--c1-seeds", type=int, default=100)
-    ap.add_argument("--tune-queries", type=int, default=128)
+    ap.add_argument("--tune-queries", type=int, default=256)
     ap.add_argument("--team-waves", type=int, default=0, help="planning launch shape (0: tuned / default)")
     ap.add_argument("--waves", type=int, default=0, help="planning grid in waves (0: tuned / resident)")
     ap.add_argument("--teams-per-query", type=int, default=0, help="teams per query (0: tuned / unbounded)")
@@ -313,7 +313,7 @@ def main():
         c_g = [q[3] for q in cal]
         for name, per_query, n in (("batch", False, len(cal)), ("plain", True, min(32, len(cal)))):
             best, cands = native.rrtc_autotune(c_rob[:n], c_sc[:n], c_s[:n], c_g[:n], native.default_params(seed=77),
-                                               per_query=per_query, reps=2)
+                                               per_query=per_query, reps=2 if per_query else 4)
             tune[name] = {"team_waves": best.team_waves, "waves": best.waves, "teams_per_query": best.teams_per_query,
                           "lanes_per_iteration": best.lanes_per_iteration, "calibration_queries": n,
                           "candidates": cands}
