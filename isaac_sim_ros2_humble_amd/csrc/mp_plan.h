@@ -117,6 +117,9 @@ constexpr int kDefaultTeamWaves = 4;
 // Iterations a team runs on the query its ticket picked (solve paths; the server uses 1).
 constexpr int kItersPerTicket = 4;
 constexpr int kDefaultLanes = 64;  // lanes per iteration when neither given nor tuned
+// admission workgroups: four waves whatever the planning team size (the start/goal check gives
+// the same verdicts for any team size, and more threads empty a reused slot sooner)
+constexpr unsigned kInitBlock = 256;
 // control-block flag of the paired iteration: lanes per iteration 32 (max_step <= 32 checked)
 inline int pair_flag(int lanes) { return lanes == 32 ? 1 : 0; }
 // Resident teams of `team_waves` waves the planning kernel can keep on the device.

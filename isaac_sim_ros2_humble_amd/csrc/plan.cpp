@@ -192,7 +192,7 @@ void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params
     MP_HIP(hipMemcpyAsync(pool.dactive, h + 16, (size_t)nq * sizeof(int), hipMemcpyHostToDevice, s));
     PlanCtlHost ctl = pool.ctl(p.resolution, p.max_step);
     void* args[] = {&ctl};
-    launch(mod.fn("mp_plan_init"), (unsigned)nq, block, s, args, "plan_init");
+    launch(mod.fn("mp_plan_init"), (unsigned)nq, kInitBlock, s, args, "plan_init");
     launch(mod.fn("mp_plan"), (unsigned)teams, block, s, args, "plan");
     MP_HIP(hipMemcpyAsync(pool.hst_pinned, pool.dst, (size_t)nq * sizeof(QueryStateHost), hipMemcpyDeviceToHost, s));
     MP_HIP(hipStreamSynchronize(s));

@@ -106,7 +106,8 @@ void mp_server::step(const std::vector<int>& list, int n_new) {
             MP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             MP_HIP(hipMemcpyAsync(pool->dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, stream));
             MP_HIP(hipMemcpyAsync(pool->dactive, h + 16, (size_t)nslots * sizeof(int), hipMemcpyHostToDevice, stream));
-            MP_HIP(hipModuleLaunchKernel(mod->fn("mp_plan_init"), nslots, 1, 1, 64 * team, 1, 1, 0, stream, args, nullptr));
+            MP_HIP(hipModuleLaunchKernel(mod->fn("mp_plan_init"), nslots, 1, 1, mp::kInitBlock, 1, 1, 0, stream, args,
+                                         nullptr));
             MP_HIP(hipModuleLaunchKernel(mod->fn("mp_plan"), teams, 1, 1, 64 * team, 1, 1, 0, stream, args, nullptr));
             MP_HIP(hipMemcpyAsync(pool->hst_pinned, pool->dst, (size_t)nslots * sizeof(mp::QueryStateHost),
                                   hipMemcpyDeviceToHost, stream));
@@ -119,7 +120,7 @@ void mp_server::step(const std::vector<int>& list, int n_new) {
         void* args[] = {&ctl};
         MP_HIP(hipMemcpyAsync(pool->dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, stream));
         MP_HIP(hipMemcpyAsync(pool->dactive, h + 16, list.size() * sizeof(int), hipMemcpyHostToDevice, stream));
-        if (n_new > 0) mp::launch(mod->fn("mp_plan_init"), (unsigned)n_new, 64u * team, stream, args, "plan_init");
+        if (n_new > 0) mp::launch(mod->fn("mp_plan_init"), (unsigned)n_new, mp::kInitBlock, stream, args, "plan_init");
         mp::launch(mod->fn("mp_plan"), (unsigned)teams, 64u * team, stream, args, "plan");
         MP_HIP(hipMemcpyAsync(pool->hst_pinned, pool->dst, (size_t)nslots * sizeof(mp::QueryStateHost),
                               hipMemcpyDeviceToHost, stream));
