@@ -29,15 +29,16 @@ def make_queries(models, n, seed):
     return workloads.c4_batch(models, n, check, edge, seed=seed)
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_served_equals_solo_exclusive(fr3, fetch, use_graph):
+@pytest.mark.parametrize("use_graph,lanes", [(False, 64), (True, 64), (True, 32)])
+def test_served_equals_solo_exclusive(fr3, fetch, use_graph, lanes):
     models = [fr3, fetch]
     qs = make_queries(models, 12, seed=21)
     scenes = [native.Scene(q[1]) for q in qs]
-    solo = [native.rrtc_solve(robot(models[k]), scenes[i], s, g, native.default_params(exclusive=1, seed=100 + i))
+    solo = [native.rrtc_solve(robot(models[k]), scenes[i], s, g,
+                              native.default_params(exclusive=1, seed=100 + i, lanes_per_iteration=lanes))
             for i, (k, _, s, g, _) in enumerate(qs)]
     srv = native.Server([robot(fr3), robot(fetch)], tree_memory_bytes=1 << 30, max_active=5, step_iterations=7,
-                        use_graph=use_graph, params=native.default_params(exclusive=1))
+                        use_graph=use_graph, params=native.default_params(exclusive=1, lanes_per_iteration=lanes))
     tickets = []
     for i, (k, _, s, g, _) in enumerate(qs):
         tickets.append(srv.submit(k, scenes[i], s, g, seed=100 + i))
@@ -54,12 +55,14 @@ def test_served_equals_solo_exclusive(fr3, fetch, use_graph):
     srv.close()
 
 
-def test_served_shared_mode_results_valid(fr3, fetch):
+@pytest.mark.parametrize("lanes", [0, 32])
+def test_served_shared_mode_results_valid(fr3, fetch, lanes):
     import oracle
     models = [fr3, fetch]
     qs = make_queries(models, 24, seed=5)
     scenes = [native.Scene(q[1]) for q in qs]
-    srv = native.Server([robot(fr3), robot(fetch)], tree_memory_bytes=2 << 30, max_active=16, use_graph=True)
+    srv = native.Server([robot(fr3), robot(fetch)], tree_memory_bytes=2 << 30, max_active=16, use_graph=True,
+                        params=native.default_params(lanes_per_iteration=lanes))
     tickets = [srv.submit(k, scenes[i], s, g, seed=i) for i, (k, _, s, g, _) in enumerate(qs)]
     srv.wait_idle(120.0)
     nsol, nvalid = 0, 0
