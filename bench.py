@@ -288,7 +288,8 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: queries shard across GPUs with no exchange, so the "
-              "single-GPU number is the per-GPU share (see DESIGN.md)", file=sys.stderr)
+              "single-GPU number is the per-GPU share of an N-GPU job (one process per GPU, each "
+              "with its own queries, scenes and trees)", file=sys.stderr)
         sys.exit(2)
 
     native.init(0)
