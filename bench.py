@@ -221,7 +221,7 @@ def bench_c3(robot, model, seeds):
             "p99_ms": float(np.percentile(times, 99)), "obstacles": scene.num_obstacles}
 
 
-def bench_mbm(models):
+def bench_mbm(models, sampler=0):
     """MotionBenchMaker problems (committed fixtures: 5 per set, 14 sets) in one batched launch."""
     from isaac_sim_ros2_humble_amd import mbm
     mm = mbm.problem_models()
@@ -229,7 +229,7 @@ def bench_mbm(models):
     probs = mbm.load_problems(os.path.join(REPO, "tests", "golden", "mbm"), mm)
     scenes = [native.Scene(p[2]) for p in probs]
     res, ms, kms = run_batch([rob[p[1]] for p in probs], scenes, [p[3] for p in probs], [p[4] for p in probs],
-                             native.default_params(seed=0))
+                             native.default_params(seed=0, sampler=sampler))
     by = {}
     for p, r in zip(probs, res):
         d = by.setdefault(p[0], [0, 0])
@@ -284,6 +284,8 @@ def main():
     ap.add_argument("--waves", type=int, default=0, help="planning grid in waves (0: tuned / resident)")
     ap.add_argument("--teams-per-query", type=int, default=0, help="teams per query (0: tuned / unbounded)")
     ap.add_argument("--lanes", type=int, default=0, help="lanes per iteration, 32 or 64 (0: tuned / default)")
+    ap.add_argument("--sampler", choices=["uniform", "halton"], default="uniform",
+                    help="sample streams of the batched planner: Philox uniform or rotated Halton")
     ap.add_argument("--skip", nargs="*", default=[], help="parts to skip: tune plain serve c2 c3 mbm cpu (cpu also skips c1)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -321,7 +323,8 @@ def main():
             log("autotune %s: team_waves %d waves %d teams_per_query %d lanes_per_iteration %d" % (
                 name, best.team_waves, best.waves, best.teams_per_query, best.lanes_per_iteration))
     params = native.default_params(seed=0, team_waves=a.team_waves, waves=a.waves, teams_per_query=a.teams_per_query,
-                                   lanes_per_iteration=a.lanes)
+                                   lanes_per_iteration=a.lanes,
+                                   sampler=native.SAMPLER_HALTON if a.sampler == "halton" else native.SAMPLER_UNIFORM)
 
     # the 256 queries are packed once (host arrays, handles of resident scenes and robots); each
     # step submits them, runs the persistent launch and has every path on the host
@@ -372,7 +375,8 @@ def main():
            "config": {"workload": "batched planning: %d independent RRT-Connect queries, FR3 7-DOF / Fetch 8-DOF "
                                   "mixed, 5-200 obstacles (box/cylinder/sphere), half straight-line-free, tree "
                                   "capacity 65536, resolution 0.02, one persistent launch per step" % len(queries),
-                      "queries_per_step": len(queries), "tree_capacity": 65536, "resolution": 0.02},
+                      "queries_per_step": len(queries), "tree_capacity": 65536, "resolution": 0.02,
+                      "sampler": a.sampler},
            "batched": {"solved_per_step": solved, "success_rate": float(np.mean(solved)) / len(queries),
                        "plan_kernel_ms": plan_kernel_ms, "configs_checked_per_step": float(np.mean(checked)),
                        "host_ms": float(np.mean(host_ms)), "iterations_p50_p99_max": [
@@ -421,7 +425,7 @@ def main():
         log("c3:", out["c3"])
 
     if "mbm" not in a.skip:
-        out["mbm"] = bench_mbm(models)
+        out["mbm"] = bench_mbm(models, params.sampler)
         log("mbm:", out["mbm"])
 
     if "cpu" not in a.skip:

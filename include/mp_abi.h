@@ -176,6 +176,12 @@ int mp_nn_argmin(const float* d_tree_soa, int n_nodes, int capacity, int dof, co
  * (SingleArmSpace.cu:184-217, 251-322). */
 int mp_sample(const mp_robot* robot, uint64_t seed, uint64_t offset, int64_t n, float* d_q, void* stream);
 
+/* Low-discrepancy samples: q[i] = Halton point offset + i + 1 in the prime bases 3, 5, 7, ...
+ * (one per active joint), each coordinate shifted modulo 1 by a per-seed Philox rotation and
+ * scaled into the bounds.  The sampler of pRRTC (pRRTC.cu:43-105: halton_initialize /
+ * halton_next), made stateless so that any worker can draw any index of the stream. */
+int mp_sample_halton(const mp_robot* robot, uint64_t seed, uint64_t offset, int64_t n, float* d_q, void* stream);
+
 /* Steering: to[i] <- from[i] + (max/d)(to[i]-from[i]) when d > max (interpolate_kernel,
  * SingleArmSpace.cu:219-249). */
 int mp_steer(const float* d_from, float* d_to, int n, int dof, float max_distance, void* stream);
@@ -203,10 +209,16 @@ typedef struct {
                               32: two iterations per pass, lanes 0-31 and 32-63 (the
                               reference's 32-thread CPRRTC block, needs max_step <= 32);
                               0 = tuned default */
+    int sampler;           /* MP_SAMPLER_UNIFORM (Philox, as mp_sample; the CPRRTC sampler) or
+                              MP_SAMPLER_HALTON (as mp_sample_halton; pRRTC's sampler) */
 } mp_rrtc_params;
 
+#define MP_SAMPLER_UNIFORM 0
+#define MP_SAMPLER_HALTON 1
+
 /* Defaults: resolution 0.02, max_step 32, max_iterations 10000, tree_capacity 65536,
- * waves 0, max_path 4096, seed 0, exclusive 0, team_waves 0, teams_per_query 0. */
+ * waves 0, max_path 4096, seed 0, exclusive 0, team_waves 0, teams_per_query 0,
+ * lanes_per_iteration 0, sampler MP_SAMPLER_UNIFORM. */
 void mp_rrtc_default_params(mp_rrtc_params* p);
 
 #define MP_PLAN_RUNNING 0

@@ -731,6 +731,26 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
         o << "        }\n";
     }
     o << "    }\n";
+    // Halton sample: point idx + 1 of the Halton sequence in the prime bases 3, 5, 7, ... (one
+    // per active joint), rotated modulo 1 per seed (Cranley-Patterson) by Philox words drawn at
+    // counter (0, 0, block, 0x6861), then scaled into the bounds as above
+    static const unsigned kHaltonBases[16] = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59};
+    o << "    static __device__ __forceinline__ void sample_halton(u64 seed, u64 idx, float* q) {\n";
+    o << "        const u32 i = (u32)(idx + 1);\n";
+    for (int blk = 0; blk * 4 < m.dof; blk++) {
+        o << "        { u32 c0 = 0u, c1 = 0u, c2 = " << blk << "u, c3 = 0x6861u;\n";
+        o << "          mp_philox(c0, c1, c2, c3, (u32)seed, (u32)(seed >> 32));\n";
+        const char* cn[4] = {"c0", "c1", "c2", "c3"};
+        for (int w = 0; w < 4 && blk * 4 + w < m.dof; w++) {
+            int j = blk * 4 + w;
+            float range = m.act_hi[j] - m.act_lo[j];
+            o << "          { float v = mp_radical_inverse<" << kHaltonBases[j] << "u>(i) + (float)(" << cn[w]
+              << " >> 8) * 5.9604644775390625e-08f; v = v >= 1.0f ? v - 1.0f : v;\n";
+            o << "            q[" << j << "] = __builtin_fmaf(v, " << lit(range) << ", " << lit(m.act_lo[j]) << "); }\n";
+        }
+        o << "        }\n";
+    }
+    o << "    }\n";
     o << "};\n";
     return o.str();
 }
@@ -753,7 +773,9 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
           << "(const float* q, long long n, SceneDev sc, float* c, unsigned char* v, unsigned long long* w) { mp_cost_kernel<" << R
           << ">(q, n, sc, c, v, w); }\n";
         o << "extern \"C\" __global__ void __launch_bounds__(256) mp_sample_" << s
-          << "(u64 seed, u64 off, long long n, float* q) { mp_sample_kernel<" << R << ">(seed, off, n, q); }\n";
+          << "(u64 seed, u64 off, long long n, float* q) { mp_sample_kernel<" << R << ">(seed, off, n, q, MP_SAMPLER_UNIFORM); }\n";
+        o << "extern \"C\" __global__ void __launch_bounds__(256) mp_halton_" << s
+          << "(u64 seed, u64 off, long long n, float* q) { mp_sample_kernel<" << R << ">(seed, off, n, q, MP_SAMPLER_HALTON); }\n";
         o << "extern \"C\" __global__ void __launch_bounds__(256) mp_edges_" << s
           << "(const float* a, const float* b, int m, float res, SceneDev sc, unsigned char* ok, float* len, unsigned long long* w) { mp_edges_kernel<"
           << R << ">(a, b, m, res, sc, ok, len, w); }\n";
@@ -768,6 +790,7 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "    Q.parent[0] = mp_sld(&p->parent[0]); Q.parent[1] = mp_sld(&p->parent[1]); Q.path = mp_sld(&p->path);\n"
          "    Q.seed = mp_sld(&p->seed); Q.robot = mp_sld(&p->robot); Q.dof = mp_sld(&p->dof); Q.cap = mp_sld(&p->cap);\n"
          "    Q.max_iters = mp_sld(&p->max_iters); Q.max_path = mp_sld(&p->max_path); Q.dof_max = mp_sld(&p->dof_max);\n"
+         "    Q.sampler = mp_sld(&p->sampler);\n"
          "}\n";
     // planning kernels: one team (workgroup of W waves, W = blockDim.x / 64) per query iteration
     const std::string team_setup =

@@ -635,4 +635,15 @@ int mp_sample(const mp_robot* robot, uint64_t seed, uint64_t offset, int64_t n, 
     });
 }
 
+int mp_sample_halton(const mp_robot* robot, uint64_t seed, uint64_t offset, int64_t n, float* d_q, void* stream) {
+    return guard([&] {
+        need_init();
+        if (!robot || n < 0 || (n && !d_q)) throw Error(MP_E_INVALID, "bad arguments");
+        if (n == 0) return;
+        long long nn = n;
+        void* args[] = {&seed, &offset, &nn, &d_q};
+        mp::launch(robot->module->fn("mp_halton_0"), (unsigned)((n + 255) / 256), 256, stream_of(stream), args, "halton");
+    });
+}
+
 }  // extern "C"

@@ -27,7 +27,7 @@ struct QueryDevHost {
     float* path;
     uint64_t seed;
     int robot, dof, cap, max_iters;
-    int max_path, dof_max, pad1, pad2;
+    int max_path, dof_max, sampler, pad2;
     float start[16];
     float goal[16];
 };
@@ -89,7 +89,7 @@ struct SlotPool {
     ~SlotPool();
     size_t bytes_per_slot() const;
     void fill_query(int slot, int robot_slot, int dof, const mp_scene* scene, const float* start, const float* goal,
-                    uint64_t seed, int max_iters);
+                    uint64_t seed, int max_iters, int sampler);
     PlanCtlHost ctl(float res, int max_step) const;
 };
 

@@ -72,7 +72,7 @@ SlotPool::~SlotPool() {
 size_t SlotPool::bytes_per_slot() const { return slot_bytes(cap, dof_max); }
 
 void SlotPool::fill_query(int slot, int robot_slot, int dof, const mp_scene* scene, const float* start,
-                          const float* goal, uint64_t seed, int max_iters) {
+                          const float* goal, uint64_t seed, int max_iters, int sampler) {
     QueryDevHost& q = hq[slot];
     std::memset(&q, 0, sizeof(q));
     q.scene = scene_view(scene);
@@ -88,6 +88,7 @@ void SlotPool::fill_query(int slot, int robot_slot, int dof, const mp_scene* sce
     q.max_iters = max_iters;
     q.max_path = max_path;
     q.dof_max = dof_max;
+    q.sampler = sampler;
     for (int j = 0; j < dof; j++) {
         q.start[j] = start[j];
         q.goal[j] = goal[j];
@@ -152,6 +153,8 @@ void check_params(const mp_rrtc_params* p) {
         throw Error(MP_E_INVALID, "lanes_per_iteration must be 0, 32 or 64");
     if (p->lanes_per_iteration == 32 && p->max_step > 32)
         throw Error(MP_E_INVALID, "lanes_per_iteration 32 needs max_step <= 32");
+    if (p->sampler != MP_SAMPLER_UNIFORM && p->sampler != MP_SAMPLER_HALTON)
+        throw Error(MP_E_INVALID, "sampler must be MP_SAMPLER_UNIFORM or MP_SAMPLER_HALTON");
 }
 
 double now_ms() {
@@ -331,7 +334,7 @@ void solve_impl(const char* tag, int nq, mp_robot* const* robots, const mp_scene
     SlotPool& pool = pool_for(tag, nq, params->tree_capacity, dof_max, params->max_path);
     for (int k = 0; k < nq; k++)
         pool.fill_query(k, slot_of[k], robots[k]->model.dof, scenes[k], starts16 + 16 * k, goals16 + 16 * k,
-                        params->seed + (uint64_t)k, params->max_iterations);
+                        params->seed + (uint64_t)k, params->max_iterations, params->sampler);
     // launch shape: explicit parameters, else the tuned shape for this robot set and class, else
     // the built-in defaults
     int W = params->team_waves > 0 ? params->team_waves : kDefaultTeamWaves;
@@ -374,6 +377,7 @@ void mp_rrtc_default_params(mp_rrtc_params* p) {
     p->team_waves = 0;
     p->teams_per_query = 0;
     p->lanes_per_iteration = 0;
+    p->sampler = MP_SAMPLER_UNIFORM;
 }
 
 int mp_rrtc_solve(mp_robot* robot, const mp_scene* scene, const float* start, const float* goal,

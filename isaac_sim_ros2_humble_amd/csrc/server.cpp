@@ -144,7 +144,8 @@ void mp_server::run() {
                     int slot = free_slots.back();
                     free_slots.pop_back();
                     mp_robot* r = robots[p.robot];
-                    pool->fill_query(slot, robot_slot[p.robot], r->model.dof, p.scene, p.start, p.goal, p.seed, prm.params.max_iterations);
+                    pool->fill_query(slot, robot_slot[p.robot], r->model.dof, p.scene, p.start, p.goal, p.seed, prm.params.max_iterations,
+                                     prm.params.sampler);
                     active[slot] = ActiveQ{p.ticket, p.t_submit, r->model.dof};
                     fresh.push_back(slot);
                 }
@@ -242,6 +243,8 @@ int mp_server_create(const mp_server_params* params, mp_server** out) {
             throw Error(MP_E_INVALID, "lanes_per_iteration must be 0, 32 or 64");
         if (p.lanes_per_iteration == 32 && p.max_step > 32)
             throw Error(MP_E_INVALID, "lanes_per_iteration 32 needs max_step <= 32");
+        if (p.sampler != MP_SAMPLER_UNIFORM && p.sampler != MP_SAMPLER_HALTON)
+            throw Error(MP_E_INVALID, "sampler must be MP_SAMPLER_UNIFORM or MP_SAMPLER_HALTON");
         s->team = mp::team_size(p);
         s->cap = p.teams_per_query;
         s->lanes = p.lanes_per_iteration;

@@ -31,7 +31,10 @@ class RRTCParams(C.Structure):
     _fields_ = [("resolution", C.c_float), ("max_step", C.c_int), ("max_iterations", C.c_int),
                 ("tree_capacity", C.c_int), ("waves", C.c_int), ("max_path", C.c_int), ("seed", C.c_uint64),
                 ("exclusive", C.c_int), ("team_waves", C.c_int), ("teams_per_query", C.c_int),
-                ("lanes_per_iteration", C.c_int)]
+                ("lanes_per_iteration", C.c_int), ("sampler", C.c_int)]
+
+
+SAMPLER_UNIFORM, SAMPLER_HALTON = 0, 1
 
 
 class Path(C.Structure):
@@ -92,6 +95,7 @@ _SIGS = {
     "mp_check_edges": (C.c_int, [P, P, P, P, C.c_int, C.c_float, P, P, P]),
     "mp_nn_argmin": (C.c_int, [P, C.c_int, C.c_int, C.c_int, P, C.c_int, P, P]),
     "mp_sample": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_int64, P, P]),
+    "mp_sample_halton": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_int64, P, P]),
     "mp_steer": (C.c_int, [P, P, C.c_int, C.c_int, C.c_float, P]),
     "mp_rrtc_default_params": (None, [C.POINTER(RRTCParams)]),
     "mp_rrtc_solve": (C.c_int, [P, P, P, P, C.POINTER(RRTCParams), C.POINTER(Path)]),
@@ -349,6 +353,12 @@ def nn_argmin(tree, queries, capacity=None):
 def sample(robot, seed, offset, n):
     dq = DeviceBuffer(4 * n * robot.dof)
     check(lib().mp_sample(robot.h, seed, offset, n, dq.ptr, None), "mp_sample")
+    return dq.to_array(np.float32, (n, robot.dof))
+
+
+def sample_halton(robot, seed, offset, n):
+    dq = DeviceBuffer(4 * n * robot.dof)
+    check(lib().mp_sample_halton(robot.h, seed, offset, n, dq.ptr, None), "mp_sample_halton")
     return dq.to_array(np.float32, (n, robot.dof))
 
 

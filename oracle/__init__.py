@@ -48,6 +48,9 @@ def lib():
         L.orc_nn.argtypes = [P, C.c_int, C.c_int, P]
         L.orc_validate_path.argtypes = [P, P, P, C.c_int, C.c_float, P]
         L.orc_sample.argtypes = [P, C.c_uint64, C.c_uint64, P]
+        L.orc_sample_halton.argtypes = [P, C.c_uint64, C.c_uint64, P]
+        L.orc_radical_inverse.argtypes = [C.c_uint32, C.c_uint32]
+        L.orc_radical_inverse.restype = C.c_float
         L.orc_philox4x32.argtypes = [C.c_uint32] * 6 + [P]
         L.orc_rrtconnect.argtypes = [P, P, P, P, C.c_float, C.c_float, C.c_uint64, C.c_int, C.c_double, P, C.c_int, P]
         _lib = L
@@ -125,6 +128,12 @@ class OracleRobot:
             lib().orc_sample(self.h, seed, idx0 + k, _p(out[k]))
         return out
 
+    def sample_halton(self, seed, idx0, n):
+        out = np.zeros((n, self.dof), np.float32)
+        for k in range(n):
+            lib().orc_sample_halton(self.h, seed, idx0 + k, _p(out[k]))
+        return out
+
 
 class OracleScene:
     def __init__(self, scene):
@@ -191,6 +200,10 @@ def philox(c, k):
     out = np.zeros(4, np.uint32)
     lib().orc_philox4x32(c[0], c[1], c[2], c[3], k[0], k[1], _p(out))
     return out
+
+
+def radical_inverse(i, b):
+    return float(lib().orc_radical_inverse(i, b))
 
 
 def full_flops(robot, scene):

@@ -738,6 +738,38 @@ void orc_sample(const orc_robot* r, uint64_t seed, uint64_t idx, float* q) {
     }
 }
 
+/* Halton sample (pRRTC.cu:43-105 draws a stateful Halton sequence per worker over the prime
+ * bases 3, 5, 7, ... of its table at :60-63).  Stateless form: sample idx is the Halton point
+ * i = idx + 1, dimension j the radical inverse of i in base ORC_HALTON_BASES[j], exact in
+ * integers (digits reversed into num over den = b^digits) and rounded once by a float division.
+ * The point set is randomised per seed by a Cranley-Patterson rotation (shift every coordinate
+ * by r_j in [0, 1), modulo 1), r_j from the Philox stream of the seed at counter (0, 0, j/4,
+ * 0x6861); coordinates are scaled into the joint bounds as in orc_sample. */
+static const uint32_t ORC_HALTON_BASES[16] = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59};
+
+float orc_radical_inverse(uint32_t i, uint32_t b) {
+    uint64_t num = 0, den = 1;
+    while (i) {
+        num = num * b + i % b;
+        den *= b;
+        i /= b;
+    }
+    return (float)num / (float)den;
+}
+
+void orc_sample_halton(const orc_robot* r, uint64_t seed, uint64_t idx, float* q) {
+    uint32_t w[4];
+    const uint32_t i = (uint32_t)(idx + 1);
+    for (int j = 0; j < r->dof; j++) {
+        if ((j & 3) == 0)
+            orc_philox4x32(0u, 0u, (uint32_t)(j >> 2), 0x6861u, (uint32_t)seed, (uint32_t)(seed >> 32), w);
+        const float rot = (float)(w[j & 3] >> 8) * (1.0f / 16777216.0f);
+        float v = orc_radical_inverse(i, ORC_HALTON_BASES[j]) + rot;
+        if (v >= 1.0f) v -= 1.0f;
+        q[j] = fmaf(v, r->act_hi[j] - r->act_lo[j], r->act_lo[j]);
+    }
+}
+
 /* ----------------------------------------------------------------------------------------- */
 /* RRT-Connect (OMPL semantics)                                                               */
 /* ----------------------------------------------------------------------------------------- */

@@ -92,6 +92,11 @@ int orc_validate_path(const orc_robot* r, const orc_scene* s, const float* path,
 /* Philox4x32-10 uniform sample: config index idx of stream (seed): lo + u*(hi-lo) per active
  * joint, u = (x >> 8) * 2^-24 with x the Philox output word for dimension j. */
 void orc_sample(const orc_robot* r, uint64_t seed, uint64_t idx, float* q);
+/* Halton sample (pRRTC.cu:43-105, stateless): radical inverse of idx + 1 in the prime base of
+ * each active joint (3, 5, 7, ...), shifted modulo 1 by a per-seed Philox rotation, scaled into
+ * the bounds.  orc_radical_inverse(i, b) = (digits of i in base b, reversed) / b^digits. */
+void orc_sample_halton(const orc_robot* r, uint64_t seed, uint64_t idx, float* q);
+float orc_radical_inverse(uint32_t i, uint32_t b);
 void orc_philox4x32(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
                     uint32_t out[4]);
 

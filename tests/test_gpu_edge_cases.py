@@ -83,7 +83,8 @@ def test_single_query_batch_and_empty_batch(fr3):
 
 @pytest.mark.parametrize("field,value", [("team_waves", 3), ("team_waves", 8), ("max_step", 65), ("max_step", 0),
                                          ("teams_per_query", -1), ("resolution", 0.0), ("tree_capacity", 1),
-                                         ("lanes_per_iteration", 16), ("lanes_per_iteration", 48)])
+                                         ("lanes_per_iteration", 16), ("lanes_per_iteration", 48),
+                                         ("sampler", 2), ("sampler", -1)])
 def test_bad_params_rejected(fr3, field, value):
     q = np.zeros(7, np.float32)
     p = native.default_params()

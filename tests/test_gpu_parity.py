@@ -132,6 +132,14 @@ def test_sample_bit_exact(name, gpu_robots, request):
     assert np.array_equal(g, orc.sample(0x1234567890, 77, 3000))
 
 
+@pytest.mark.parametrize("name", ["fr3", "fetch"])
+def test_sample_halton_bit_exact(name, gpu_robots, request):
+    orc = request.getfixturevalue("orc_" + name)
+    for seed, off, n in ((0x1234567890, 0, 3000), (5, 123456, 500), (9, 2**32 - 4, 8)):  # last: index wraps to 0
+        g = native.sample_halton(gpu_robots[name], seed=seed, offset=off, n=n)
+        assert np.array_equal(g, orc.sample_halton(seed, off, n)), (seed, off)
+
+
 def test_nn_argmin(rng):
     tree = rng.normal(size=(5000, 8)).astype(np.float32)
     tree[4000] = tree[17]  # duplicate node: ties -> lowest index

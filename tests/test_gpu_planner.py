@@ -197,6 +197,45 @@ def test_batch_shared_mixed_robots(fr3, fetch, lanes):
 
 
 @pytest.mark.parametrize("lanes", [64, 32])
+def test_halton_sampler(fr3, fetch, lanes):
+    """Halton sample streams: shared-mode batches solve and keep the planner's contract;
+    exclusive mode stays deterministic per seed with batch == solo; the streams differ from
+    the uniform sampler's (the setting reaches the kernel)."""
+    rng = np.random.default_rng(31)
+    queries = []
+    for k in range(12):
+        m = fr3 if k % 2 == 0 else fetch
+        scene = workloads.random_scene(rng, workloads.log_uniform_count(rng, 5, 60), *workloads.workspace(m))
+        pair = workloads.sample_pair(m, scene, rng, gpu_check, gpu_edge, want_free_line=False)
+        if pair is not None:
+            queries.append((m, scene, pair[0], pair[1]))
+    assert len(queries) >= 8
+    robots = [_robot(m) for m, _, _, _ in queries]
+    scenes = [native.Scene(sc) for _, sc, _, _ in queries]
+    starts, goals = [s for _, _, s, _ in queries], [g for _, _, _, g in queries]
+    H = native.SAMPLER_HALTON
+    res = native.rrtc_solve_batch(robots, scenes, starts, goals,
+                                  native.default_params(seed=3, lanes_per_iteration=lanes, sampler=H))
+    solved = [k for k, r in enumerate(res) if r["status"] == "solved"]
+    assert len(solved) >= 0.8 * len(res)
+    valid = sum(_assert_valid(queries[k][0], queries[k][1], res[k], starts[k], goals[k]) for k in solved)
+    assert valid >= 0.9 * len(solved)
+    ex = native.default_params(exclusive=1, seed=7, lanes_per_iteration=lanes, sampler=H)
+    batch = native.rrtc_solve_batch(robots[:4], scenes[:4], starts[:4], goals[:4], ex)
+    uni = native.rrtc_solve_batch(robots[:4], scenes[:4], starts[:4], goals[:4],
+                                  native.default_params(exclusive=1, seed=7, lanes_per_iteration=lanes))
+    differs = False
+    for k in range(4):
+        solo = native.rrtc_solve(robots[k], scenes[k], starts[k], goals[k],
+                                 native.default_params(exclusive=1, seed=7 + k, lanes_per_iteration=lanes, sampler=H))
+        assert solo["status"] == batch[k]["status"] and np.array_equal(solo["path"], batch[k]["path"])
+        if solo["status"] == "solved":
+            _assert_valid(queries[k][0], queries[k][1], solo, starts[k], goals[k])
+        differs |= solo["iterations"] != uni[k]["iterations"] or not np.array_equal(solo["path"], uni[k]["path"])
+    assert differs
+
+
+@pytest.mark.parametrize("lanes", [64, 32])
 def test_reused_slots_hold_no_earlier_nodes(fr3, fetch, lanes):
     """Admission empties every coordinate row of a reused slot: batches of the same size run in
     the same slot pool, here with the robots swapped between slots from one batch to the next

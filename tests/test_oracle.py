@@ -44,6 +44,49 @@ def test_samples_in_bounds_and_uniform(fr3, orc_fr3):
     assert np.array_equal(orc_fr3.sample(3, 100, 5), q[100:105])
 
 
+def test_radical_inverse_closed_forms():
+    # van der Corput points written out by hand: i in base b, digits mirrored behind the point
+    base3 = [0, 1 / 3, 2 / 3, 1 / 9, 4 / 9, 7 / 9, 2 / 9, 5 / 9, 8 / 9, 1 / 27]
+    for i, v in enumerate(base3):
+        assert oracle.radical_inverse(i, 3) == np.float32(v), i
+    # 6 = 11 in base 5 -> 0.11 = 6/25; 7 = 12 -> 0.21 = 11/25; 26 = 101 -> 0.101 = 26/125
+    assert oracle.radical_inverse(6, 5) == np.float32(6 / 25)
+    assert oracle.radical_inverse(7, 5) == np.float32(11 / 25)
+    assert oracle.radical_inverse(26, 5) == np.float32(26 / 125)
+    assert oracle.radical_inverse(58, 59) == np.float32(58 / 59)
+
+
+@pytest.mark.parametrize("b", [3, 5, 7, 11, 13, 17, 19, 23])
+def test_radical_inverse_stratifies(b):
+    # the first b^k points of a van der Corput sequence are exactly the grid m / b^k (a
+    # permutation of it): no digit dropped, none out of order
+    for k in (1, 2, 3):
+        pts = sorted(oracle.radical_inverse(i, b) for i in range(b ** k))
+        assert pts == [float(np.float32(m / b ** k)) for m in range(b ** k)], (b, k)
+
+
+def test_halton_samples_rotated_grid(fr3, orc_fr3):
+    # per joint j (base b_j = 3, 5, 7, ...): the points idx = 0 .. b^2 - 2 (Halton i = 1 .. b^2 - 1)
+    # fill b^2 - 1 distinct cells of a b^2 grid, all at the same offset within their cells (one
+    # rotation per seed), and stay inside the bounds
+    bases = [3, 5, 7, 11, 13, 17, 19]
+    lo, hi = fr3.active_lower.astype(np.float64), fr3.active_upper.astype(np.float64)
+    for seed in (0, 99):
+        q = orc_fr3.sample_halton(seed, 0, 19 * 19 - 1).astype(np.float64)
+        assert np.all(q >= lo - 1e-6) and np.all(q <= hi + 1e-6)
+        u = (q - lo) / (hi - lo)
+        for j, b in enumerate(bases):
+            x = u[: b * b - 1, j] * b * b
+            cells, off = np.floor(x + 1e-4), x - np.floor(x + 1e-4)
+            assert len(np.unique(cells)) == b * b - 1
+            d = (off - off[0] + 0.5) % 1.0 - 0.5  # offsets compared modulo 1
+            # float32 rounding of the sample (~1e-7 of the range) scaled by b^2 <= 361
+            assert np.abs(d).max() < 3e-4, (seed, j)
+    # the rotation depends on the seed; the stream is counter based
+    assert not np.array_equal(orc_fr3.sample_halton(0, 0, 4), orc_fr3.sample_halton(1, 0, 4))
+    assert np.array_equal(orc_fr3.sample_halton(7, 40, 3), orc_fr3.sample_halton(7, 0, 43)[40:])
+
+
 # ---------------------------------------------------------------------------------------- FK64
 def test_fr3_zero_pose_closed_form(fr3, orc_fr3):
     # Franka zero pose: flange at (0.088, 0, 0.333+0.316+0.384-0.107) = (0.088, 0, 0.926), z-axis down
