@@ -221,7 +221,7 @@ def bench_c3(robot, model, seeds):
             "p99_ms": float(np.percentile(times, 99)), "obstacles": scene.num_obstacles}
 
 
-def bench_mbm(models, sampler=0):
+def bench_mbm(models, sampler=0, dynamic_domain=0):
     """MotionBenchMaker problems (committed fixtures: 5 per set, 14 sets) in one batched launch."""
     from isaac_sim_ros2_humble_amd import mbm
     mm = mbm.problem_models()
@@ -229,7 +229,7 @@ def bench_mbm(models, sampler=0):
     probs = mbm.load_problems(os.path.join(REPO, "tests", "golden", "mbm"), mm)
     scenes = [native.Scene(p[2]) for p in probs]
     res, ms, kms = run_batch([rob[p[1]] for p in probs], scenes, [p[3] for p in probs], [p[4] for p in probs],
-                             native.default_params(seed=0, sampler=sampler))
+                             native.default_params(seed=0, sampler=sampler, dynamic_domain=dynamic_domain))
     by = {}
     for p, r in zip(probs, res):
         d = by.setdefault(p[0], [0, 0])
@@ -286,6 +286,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=0, help="lanes per iteration, 32 or 64 (0: tuned / default)")
     ap.add_argument("--sampler", choices=["uniform", "halton"], default="uniform",
                     help="sample streams of the batched planner: Philox uniform or rotated Halton")
+    ap.add_argument("--dynamic-domain", action="store_true",
+                    help="dynamic-domain sampling in the batched planner (pRRTC's settings)")
     ap.add_argument("--skip", nargs="*", default=[], help="parts to skip: tune plain serve c2 c3 mbm cpu (cpu also skips c1)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -324,7 +326,8 @@ def main():
                 name, best.team_waves, best.waves, best.teams_per_query, best.lanes_per_iteration))
     params = native.default_params(seed=0, team_waves=a.team_waves, waves=a.waves, teams_per_query=a.teams_per_query,
                                    lanes_per_iteration=a.lanes,
-                                   sampler=native.SAMPLER_HALTON if a.sampler == "halton" else native.SAMPLER_UNIFORM)
+                                   sampler=native.SAMPLER_HALTON if a.sampler == "halton" else native.SAMPLER_UNIFORM,
+                                   dynamic_domain=1 if a.dynamic_domain else 0)
 
     # the 256 queries are packed once (host arrays, handles of resident scenes and robots); each
     # step submits them, runs the persistent launch and has every path on the host
@@ -376,7 +379,7 @@ def main():
                                   "mixed, 5-200 obstacles (box/cylinder/sphere), half straight-line-free, tree "
                                   "capacity 65536, resolution 0.02, one persistent launch per step" % len(queries),
                       "queries_per_step": len(queries), "tree_capacity": 65536, "resolution": 0.02,
-                      "sampler": a.sampler},
+                      "sampler": a.sampler, "dynamic_domain": bool(a.dynamic_domain)},
            "batched": {"solved_per_step": solved, "success_rate": float(np.mean(solved)) / len(queries),
                        "plan_kernel_ms": plan_kernel_ms, "configs_checked_per_step": float(np.mean(checked)),
                        "host_ms": float(np.mean(host_ms)), "iterations_p50_p99_max": [
@@ -425,7 +428,7 @@ def main():
         log("c3:", out["c3"])
 
     if "mbm" not in a.skip:
-        out["mbm"] = bench_mbm(models, params.sampler)
+        out["mbm"] = bench_mbm(models, params.sampler, params.dynamic_domain)
         log("mbm:", out["mbm"])
 
     if "cpu" not in a.skip:

@@ -211,6 +211,15 @@ typedef struct {
                               0 = tuned default */
     int sampler;           /* MP_SAMPLER_UNIFORM (Philox, as mp_sample; the CPRRTC sampler) or
                               MP_SAMPLER_HALTON (as mp_sample_halton; pRRTC's sampler) */
+    int dynamic_domain;    /* 1: dynamic-domain sampling, pRRTC's (pRRTC.cu:420, 456-468, 584-598):
+                              a sample farther from its nearest node than that node's radius is
+                              dropped; a node's radius is unbounded until an extension from it
+                              fails, then dd_radius, shrunk by (1 - dd_alpha) on each later
+                              failure (not below dd_min_radius) and grown by (1 + dd_alpha) on
+                              each success */
+    float dd_radius;       /* rad (pRRTC 4.0) */
+    float dd_alpha;        /* (pRRTC 1e-4) */
+    float dd_min_radius;   /* rad (pRRTC 1.0) */
 } mp_rrtc_params;
 
 #define MP_SAMPLER_UNIFORM 0
@@ -218,7 +227,8 @@ typedef struct {
 
 /* Defaults: resolution 0.02, max_step 32, max_iterations 10000, tree_capacity 65536,
  * waves 0, max_path 4096, seed 0, exclusive 0, team_waves 0, teams_per_query 0,
- * lanes_per_iteration 0, sampler MP_SAMPLER_UNIFORM. */
+ * lanes_per_iteration 0, sampler MP_SAMPLER_UNIFORM, dynamic_domain 0 (dd_radius 4, dd_alpha 1e-4,
+ * dd_min_radius 1). */
 void mp_rrtc_default_params(mp_rrtc_params* p);
 
 #define MP_PLAN_RUNNING 0

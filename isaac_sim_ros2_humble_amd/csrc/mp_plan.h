@@ -57,6 +57,8 @@ struct PlanCtlHost {
     float res;
     int max_step;
     unsigned long long* prof;  // planner phase profile counters, null when off
+    float dd_radius, dd_alpha, dd_min;  // dynamic-domain settings
+    int dd;                             // 1: dynamic domain on
 };
 
 hipStream_t lib_stream();
@@ -90,10 +92,11 @@ struct SlotPool {
     size_t bytes_per_slot() const;
     void fill_query(int slot, int robot_slot, int dof, const mp_scene* scene, const float* start, const float* goal,
                     uint64_t seed, int max_iters, int sampler);
-    PlanCtlHost ctl(float res, int max_step) const;
+    PlanCtlHost ctl(const mp_rrtc_params& p) const;
 };
 
 size_t slot_bytes(int cap, int dof_max);  // device bytes of one slot
+void check_domain(const mp_rrtc_params& p);  // dynamic-domain settings (throws MP_E_INVALID)
 
 // Runs f, converting exceptions to an ABI status code (message in mp_last_error()).
 int guard_call(const std::function<void()>& f);

@@ -16,13 +16,14 @@ using mp::Error;
 namespace mp {
 
 size_t slot_bytes(int cap, int dof_max) {
-    return (size_t)2 * dof_max * cap * sizeof(float) + (size_t)2 * cap * sizeof(int);
+    // per tree: coordinates [dof_max][cap], then parents int[cap] and domain radii float[cap]
+    return (size_t)2 * dof_max * cap * sizeof(float) + (size_t)2 * cap * (sizeof(int) + sizeof(float));
 }
 
 SlotPool::SlotPool(int ns, int c, int d, int mp_) : nslots(ns), cap(c), dof_max(d), max_path(mp_) {
     try {
         MP_HIP(hipMalloc(&trees, (size_t)nslots * 2 * dof_max * cap * sizeof(float)));
-        MP_HIP(hipMalloc(&parents, (size_t)nslots * 2 * cap * sizeof(int)));
+        MP_HIP(hipMalloc(&parents, (size_t)nslots * 2 * 2 * cap * sizeof(int)));  // parents, radii
         MP_HIP(hipHostMalloc(&hpaths, (size_t)nslots * max_path * dof_max * sizeof(float),
                              hipHostMallocMapped | hipHostMallocCoherent));
         MP_HIP(hipHostGetDevicePointer((void**)&paths, hpaths, 0));
@@ -34,9 +35,10 @@ SlotPool::SlotPool(int ns, int c, int d, int mp_) : nslots(ns), cap(c), dof_max(
         MP_HIP(hipHostMalloc(&hst_pinned, (size_t)nslots * sizeof(QueryStateHost), hipHostMallocDefault));
         MP_HIP(hipHostMalloc(&hq, (size_t)nslots * sizeof(QueryDevHost), hipHostMallocDefault));
         MP_HIP(hipMemset(dst, 0, (size_t)nslots * sizeof(QueryStateHost)));
-        // every tree slot starts empty: NaN coordinates (0x7fc0dead) and parent -2
+        // every tree slot starts empty: NaN coordinates (0x7fc0dead) and parent -2 (the radii read
+        // 0xfffffffe, a NaN: an unbounded domain)
         MP_HIP(hipMemsetD32((hipDeviceptr_t)trees, (int)0x7fc0deadu, (size_t)nslots * 2 * dof_max * cap));
-        MP_HIP(hipMemsetD32((hipDeviceptr_t)parents, -2, (size_t)nslots * 2 * cap));
+        MP_HIP(hipMemsetD32((hipDeviceptr_t)parents, -2, (size_t)nslots * 2 * 2 * cap));
         MP_HIP(hipDeviceSynchronize());
     } catch (...) {
         this->~SlotPool();
@@ -78,7 +80,7 @@ void SlotPool::fill_query(int slot, int robot_slot, int dof, const mp_scene* sce
     q.scene = scene_view(scene);
     for (int t = 0; t < 2; t++) {
         q.tree[t] = trees + ((size_t)slot * 2 + t) * dof_max * cap;
-        q.parent[t] = parents + ((size_t)slot * 2 + t) * cap;
+        q.parent[t] = parents + ((size_t)slot * 2 + t) * 2 * cap;  // radii follow at + cap
     }
     q.path = paths + (size_t)slot * max_path * dof_max;
     q.seed = seed;
@@ -95,15 +97,19 @@ void SlotPool::fill_query(int slot, int robot_slot, int dof, const mp_scene* sce
     }
 }
 
-PlanCtlHost SlotPool::ctl(float res, int max_step) const {
+PlanCtlHost SlotPool::ctl(const mp_rrtc_params& p) const {
     PlanCtlHost c;
     c.q = dq;
     c.st = dst;
     c.active = dactive;
     c.ctl = dctl;
-    c.res = res;
-    c.max_step = max_step;
+    c.res = p.resolution;
+    c.max_step = p.max_step;
     c.prof = lib_plan_profile();
+    c.dd_radius = p.dd_radius;
+    c.dd_alpha = p.dd_alpha;
+    c.dd_min = p.dd_min_radius;
+    c.dd = p.dynamic_domain;
     return c;
 }
 
@@ -116,6 +122,12 @@ int default_teams(hipFunction_t plan_fn, int team_waves) {
 }
 
 int team_size(const mp_rrtc_params& p) { return p.team_waves > 0 ? p.team_waves : kDefaultTeamWaves; }
+
+void check_domain(const mp_rrtc_params& p) {
+    if (p.dynamic_domain != 0 && p.dynamic_domain != 1) throw Error(MP_E_INVALID, "dynamic_domain must be 0 or 1");
+    if (p.dynamic_domain && !(p.dd_radius > 0.0f && p.dd_min_radius > 0.0f && p.dd_alpha >= 0.0f && p.dd_alpha < 1.0f))
+        throw Error(MP_E_INVALID, "dynamic domain needs dd_radius > 0, dd_min_radius > 0 and 0 <= dd_alpha < 1");
+}
 
 namespace {
 std::mutex g_pool_mu;
@@ -155,6 +167,7 @@ void check_params(const mp_rrtc_params* p) {
         throw Error(MP_E_INVALID, "lanes_per_iteration 32 needs max_step <= 32");
     if (p->sampler != MP_SAMPLER_UNIFORM && p->sampler != MP_SAMPLER_HALTON)
         throw Error(MP_E_INVALID, "sampler must be MP_SAMPLER_UNIFORM or MP_SAMPLER_HALTON");
+    check_domain(*p);
 }
 
 double now_ms() {
@@ -193,7 +206,7 @@ void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params
     for (int k = 0; k < nq; k++) h[16 + k] = k;
     MP_HIP(hipMemcpyAsync(pool.dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, s));
     MP_HIP(hipMemcpyAsync(pool.dactive, h + 16, (size_t)nq * sizeof(int), hipMemcpyHostToDevice, s));
-    PlanCtlHost ctl = pool.ctl(p.resolution, p.max_step);
+    PlanCtlHost ctl = pool.ctl(p);
     void* args[] = {&ctl};
     launch(mod.fn("mp_plan_init"), (unsigned)nq, kInitBlock, s, args, "plan_init");
     launch(mod.fn("mp_plan"), (unsigned)teams, block, s, args, "plan");
@@ -378,6 +391,10 @@ void mp_rrtc_default_params(mp_rrtc_params* p) {
     p->teams_per_query = 0;
     p->lanes_per_iteration = 0;
     p->sampler = MP_SAMPLER_UNIFORM;
+    p->dynamic_domain = 0;  // pRRTC's settings (pRRTC_settings.hh:13-16) when turned on
+    p->dd_radius = 4.0f;
+    p->dd_alpha = 1e-4f;
+    p->dd_min_radius = 1.0f;
 }
 
 int mp_rrtc_solve(mp_robot* robot, const mp_scene* scene, const float* start, const float* goal,

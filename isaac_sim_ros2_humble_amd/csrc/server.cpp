@@ -101,7 +101,7 @@ void mp_server::step(const std::vector<int>& list, int n_new) {
     }
     if (prm.use_graph) {
         if (!graph) {
-            mp::PlanCtlHost ctl = pool->ctl(prm.params.resolution, prm.params.max_step);
+            mp::PlanCtlHost ctl = pool->ctl(prm.params);
             void* args[] = {&ctl};
             MP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             MP_HIP(hipMemcpyAsync(pool->dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, stream));
@@ -116,7 +116,7 @@ void mp_server::step(const std::vector<int>& list, int n_new) {
         }
         MP_HIP(hipGraphLaunch(graph, stream));
     } else {
-        mp::PlanCtlHost ctl = pool->ctl(prm.params.resolution, prm.params.max_step);
+        mp::PlanCtlHost ctl = pool->ctl(prm.params);
         void* args[] = {&ctl};
         MP_HIP(hipMemcpyAsync(pool->dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, stream));
         MP_HIP(hipMemcpyAsync(pool->dactive, h + 16, list.size() * sizeof(int), hipMemcpyHostToDevice, stream));
@@ -245,6 +245,7 @@ int mp_server_create(const mp_server_params* params, mp_server** out) {
             throw Error(MP_E_INVALID, "lanes_per_iteration 32 needs max_step <= 32");
         if (p.sampler != MP_SAMPLER_UNIFORM && p.sampler != MP_SAMPLER_HALTON)
             throw Error(MP_E_INVALID, "sampler must be MP_SAMPLER_UNIFORM or MP_SAMPLER_HALTON");
+        mp::check_domain(p);
         s->team = mp::team_size(p);
         s->cap = p.teams_per_query;
         s->lanes = p.lanes_per_iteration;

@@ -31,7 +31,8 @@ class RRTCParams(C.Structure):
     _fields_ = [("resolution", C.c_float), ("max_step", C.c_int), ("max_iterations", C.c_int),
                 ("tree_capacity", C.c_int), ("waves", C.c_int), ("max_path", C.c_int), ("seed", C.c_uint64),
                 ("exclusive", C.c_int), ("team_waves", C.c_int), ("teams_per_query", C.c_int),
-                ("lanes_per_iteration", C.c_int), ("sampler", C.c_int)]
+                ("lanes_per_iteration", C.c_int), ("sampler", C.c_int), ("dynamic_domain", C.c_int),
+                ("dd_radius", C.c_float), ("dd_alpha", C.c_float), ("dd_min_radius", C.c_float)]
 
 
 SAMPLER_UNIFORM, SAMPLER_HALTON = 0, 1

@@ -84,7 +84,7 @@ def test_single_query_batch_and_empty_batch(fr3):
 @pytest.mark.parametrize("field,value", [("team_waves", 3), ("team_waves", 8), ("max_step", 65), ("max_step", 0),
                                          ("teams_per_query", -1), ("resolution", 0.0), ("tree_capacity", 1),
                                          ("lanes_per_iteration", 16), ("lanes_per_iteration", 48),
-                                         ("sampler", 2), ("sampler", -1)])
+                                         ("sampler", 2), ("sampler", -1), ("dynamic_domain", 2)])
 def test_bad_params_rejected(fr3, field, value):
     q = np.zeros(7, np.float32)
     p = native.default_params()
@@ -92,3 +92,14 @@ def test_bad_params_rejected(fr3, field, value):
     with pytest.raises(native.MPError) as e:
         native.rrtc_solve(_robot(fr3), native.Scene(Scene()), q, q, p)
     assert "(-1)" in str(e.value)  # MP_E_INVALID, reported as a status code
+
+
+@pytest.mark.parametrize("field,value", [("dd_radius", 0.0), ("dd_min_radius", -1.0), ("dd_alpha", 1.0),
+                                         ("dd_alpha", -0.5)])
+def test_bad_domain_settings_rejected(fr3, field, value):
+    q = np.zeros(7, np.float32)
+    p = native.default_params(dynamic_domain=1)
+    setattr(p, field, value)
+    with pytest.raises(native.MPError) as e:
+        native.rrtc_solve(_robot(fr3), native.Scene(Scene()), q, q, p)
+    assert "(-1)" in str(e.value)

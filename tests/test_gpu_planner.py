@@ -197,10 +197,11 @@ def test_batch_shared_mixed_robots(fr3, fetch, lanes):
 
 
 @pytest.mark.parametrize("lanes", [64, 32])
-def test_halton_sampler(fr3, fetch, lanes):
-    """Halton sample streams: shared-mode batches solve and keep the planner's contract;
-    exclusive mode stays deterministic per seed with batch == solo; the streams differ from
-    the uniform sampler's (the setting reaches the kernel)."""
+@pytest.mark.parametrize("option", ["halton", "dynamic_domain"])
+def test_sampling_options(fr3, fetch, lanes, option):
+    """Halton sample streams, and dynamic-domain sampling (pRRTC's settings): shared-mode
+    batches solve and keep the planner's contract; exclusive mode stays deterministic per seed
+    with batch == solo; the runs differ from the plain ones (the setting reaches the kernel)."""
     rng = np.random.default_rng(31)
     queries = []
     for k in range(12):
@@ -213,21 +214,21 @@ def test_halton_sampler(fr3, fetch, lanes):
     robots = [_robot(m) for m, _, _, _ in queries]
     scenes = [native.Scene(sc) for _, sc, _, _ in queries]
     starts, goals = [s for _, _, s, _ in queries], [g for _, _, _, g in queries]
-    H = native.SAMPLER_HALTON
+    opt = {"sampler": native.SAMPLER_HALTON} if option == "halton" else {"dynamic_domain": 1}
     res = native.rrtc_solve_batch(robots, scenes, starts, goals,
-                                  native.default_params(seed=3, lanes_per_iteration=lanes, sampler=H))
+                                  native.default_params(seed=3, lanes_per_iteration=lanes, **opt))
     solved = [k for k, r in enumerate(res) if r["status"] == "solved"]
     assert len(solved) >= 0.8 * len(res)
     valid = sum(_assert_valid(queries[k][0], queries[k][1], res[k], starts[k], goals[k]) for k in solved)
     assert valid >= 0.9 * len(solved)
-    ex = native.default_params(exclusive=1, seed=7, lanes_per_iteration=lanes, sampler=H)
+    ex = native.default_params(exclusive=1, seed=7, lanes_per_iteration=lanes, **opt)
     batch = native.rrtc_solve_batch(robots[:4], scenes[:4], starts[:4], goals[:4], ex)
     uni = native.rrtc_solve_batch(robots[:4], scenes[:4], starts[:4], goals[:4],
                                   native.default_params(exclusive=1, seed=7, lanes_per_iteration=lanes))
     differs = False
     for k in range(4):
         solo = native.rrtc_solve(robots[k], scenes[k], starts[k], goals[k],
-                                 native.default_params(exclusive=1, seed=7 + k, lanes_per_iteration=lanes, sampler=H))
+                                 native.default_params(exclusive=1, seed=7 + k, lanes_per_iteration=lanes, **opt))
         assert solo["status"] == batch[k]["status"] and np.array_equal(solo["path"], batch[k]["path"])
         if solo["status"] == "solved":
             _assert_valid(queries[k][0], queries[k][1], solo, starts[k], goals[k])
