@@ -1,7 +1,9 @@
 """CPU-side checks of the native library: it loads, exports every entry point the C ABI header
 declares, generates robot source without a device, and reports errors without a GPU."""
+import ctypes as C
 import os
 import re
+import subprocess
 
 import numpy as np
 
@@ -54,7 +56,7 @@ def test_query_descriptor_loads_every_field_the_kernels_read(fr3):
     body = src[src.index("mp_load_query("):]
     body = body[:body.index("\n}\n")]
     for f in ("tree[0]", "tree[1]", "parent[0]", "parent[1]", "path", "seed", "robot", "dof", "cap", "max_iters",
-              "max_path", "dof_max", "scene.box", "scene.sph", "scene.cyl", "scene.cap"):
+              "max_path", "dof_max", "sampler", "scene.box", "scene.sph", "scene.cyl", "scene.cap"):
         assert "Q.%s = mp_sld(&p->%s)" % (f, f) in body, f
 
 
@@ -90,3 +92,26 @@ def test_invalid_robot_rejected():
         assert "topological" in str(e)
     else:
         raise AssertionError("expected an error")
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every ctypes mirror of a C ABI struct has the header's size and field offsets (compiled
+    with the host C compiler from include/mp_abi.h)."""
+    pairs = [(native.RobotDesc, "mp_robot_desc"), (native.RRTCParams, "mp_rrtc_params"), (native.Path, "mp_path"),
+             (native.TuneResult, "mp_tune_result"), (native.ServerParams, "mp_server_params"),
+             (native.ServerStats, "mp_server_stats")]
+    body = []
+    for py, c in pairs:
+        body.append('printf("%%zu", sizeof(%s));' % c)
+        body += ['printf(" %%zu", offsetof(%s, %s));' % (c, f) for f, _ in py._fields_]
+        body.append('printf("\\n");')
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"mp_abi.h\"\nint main(void) {\n" +
+                   "\n".join(body) + "\nreturn 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    lines = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    for (py, c), line in zip(pairs, lines):
+        got = [int(x) for x in line.split()]
+        assert got[0] == C.sizeof(py), c
+        assert got[1:] == [getattr(py, f).offset for f, _ in py._fields_], c
