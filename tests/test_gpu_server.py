@@ -55,14 +55,15 @@ def test_served_equals_solo_exclusive(fr3, fetch, use_graph, lanes):
     srv.close()
 
 
-@pytest.mark.parametrize("lanes", [0, 32])
-def test_served_shared_mode_results_valid(fr3, fetch, lanes):
+@pytest.mark.parametrize("lanes,options", [(0, ""), (32, ""), (32, "halton+domain")])
+def test_served_shared_mode_results_valid(fr3, fetch, lanes, options):
     import oracle
+    opt = {"sampler": native.SAMPLER_HALTON, "dynamic_domain": 1} if options else {}
     models = [fr3, fetch]
     qs = make_queries(models, 24, seed=5)
     scenes = [native.Scene(q[1]) for q in qs]
     srv = native.Server([robot(fr3), robot(fetch)], tree_memory_bytes=2 << 30, max_active=16, use_graph=True,
-                        params=native.default_params(lanes_per_iteration=lanes))
+                        params=native.default_params(lanes_per_iteration=lanes, **opt))
     tickets = [srv.submit(k, scenes[i], s, g, seed=i) for i, (k, _, s, g, _) in enumerate(qs)]
     srv.wait_idle(120.0)
     nsol, nvalid = 0, 0
