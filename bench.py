@@ -194,8 +194,13 @@ def cpu_checker_c2(model, threads, n=1 << 20):
     t0 = time.perf_counter()
     oracle.check(orc, osc, q, nthreads=threads)
     dt = time.perf_counter() - t0
+    # untimed: the canonical-order evaluated work (self pairs, then spheres x obstacles in index
+    # order, stopping at the first hit) per configuration, on a sample of the same configurations
+    ns = min(n, 65536)
+    _, _, _, fl = oracle.check(orc, osc, q[:ns], want_flops=True, nthreads=threads)
     return {"value": n / dt, "unit": "configs/s", "cores": threads, "kind": "port",
-            "sample": "%d C2 configurations (FR3, 64 obstacles), verdict mode" % n}
+            "sample": "%d C2 configurations (FR3, 64 obstacles), verdict mode" % n,
+            "canonical_flops_per_config": float(fl.mean()), "canonical_sample": ns}
 
 
 def bench_c3(robot, model, seeds):
@@ -437,6 +442,14 @@ def main():
         log("cpu:", out["cpu_baseline"])
         out["cpu_checker_c2"] = cpu_checker_c2(models[0], threads)
         log("cpu c2:", out["cpu_checker_c2"])
+        if "c2" in out:  # C2's early-out throughput at the canonical-order work count
+            cf_ = out["cpu_checker_c2"]["canonical_flops_per_config"]
+            v = out["c2"]["verdict"]
+            v["canonical_flops_per_config"] = cf_
+            v["canonical_tflops"] = cf_ * (1 << 20) / (v["ms"] / 1e3) / 1e12
+            if "roofline_c2" in out:
+                out["roofline_c2"]["achieved_canonical"] = v["canonical_tflops"]
+                out["roofline_c2"]["frac_canonical"] = v["canonical_tflops"] / PEAK_FP32_TFLOPS
         if a.c1_seeds > 0:
             out["c1"] = bench_c1(robots["fr3"], models[0], a.c1_seeds, threads)
             log("c1:", out["c1"])
