@@ -10,6 +10,8 @@ Timed region: K steps of the batched solve, each bracketed by a device synchroni
 inputs (scenes, robot modules, query descriptors) are resident in HBM before timing starts.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--serve-seconds S] [--rate L]
+                       [--team-waves W --waves N --teams-per-query C --lanes 32|64]
+                       [--sampler uniform|halton] [--dynamic-domain] [--skip PART ...]
 """
 import argparse
 import concurrent.futures as cf
