@@ -159,6 +159,24 @@ int mp_check_states(const mp_robot* robot, const mp_scene* scene, const float* d
 int mp_state_costs(const mp_robot* robot, const mp_scene* scene, const float* d_q, int64_t n,
                    float* d_cost, uint8_t* d_verdict, void* stream);
 
+/* Per-constraint costs, full evaluation: parts [MP_COST_KINDS][n] (row k, state i at k*n + i),
+ * rows in the order below; cost_opt [n] is their sum in that order (the value mp_state_costs
+ * writes) and verdict_opt [n] is (sum == 0).  The [constraint][state] cost layout of BaseStates
+ * (cudampl/CUDAMPLib/include/base/States.h:205), filled by each constraint's
+ * computeCost (EnvConstraintSphere.cu:35-117, EnvConstraintCuboid.cu:84-171,
+ * EnvConstraintCylinder.cu:107-196, SelfCollisionConstraint.cu:52-133,
+ * BoundaryConstraint.cu:46-93); capsules have no CUDAMPLib constraint (pRRTC's
+ * sphere_capsule, pRRTC/src/planning/utils.cuh:113-127, as a penetration depth). */
+#define MP_COST_KINDS 6
+#define MP_COST_ENV_SPHERE 0
+#define MP_COST_ENV_CUBOID 1
+#define MP_COST_ENV_CYLINDER 2
+#define MP_COST_ENV_CAPSULE 3
+#define MP_COST_SELF 4
+#define MP_COST_BOUNDS 5
+int mp_constraint_costs(const mp_robot* robot, const mp_scene* scene, const float* d_q, int64_t n, float* d_parts,
+                        float* d_cost_opt, uint8_t* d_verdict_opt, void* stream);
+
 /* Edge checks with checkMotions semantics (SingleArmSpace.cu:646-848): steps =
  * floor(d/res)+1, states q0 + dir*i (i < steps-1) then q1; ok[e] = all valid, len[e] = d.
  * No host round trip, no temporary allocation. */
@@ -236,10 +254,16 @@ void mp_rrtc_default_params(mp_rrtc_params* p);
 #define MP_PLAN_EXHAUSTED 2   /* max_iterations samples used (reference: returns {}) */
 #define MP_PLAN_TREE_FULL 3
 #define MP_PLAN_INVALID 4     /* start or goal in collision / out of bounds */
+#define MP_PLAN_PATH_LOST 5   /* solved, but a tree node on the path never became readable
+                                 within the path walker's wait bound: no path is returned */
+#define MP_PLAN_PATH_TOO_LONG 6 /* solved, but the path has more waypoints than max_path or the
+                                 caller's buffer holds: num_waypoints is the length needed and
+                                 nothing is copied */
 
 typedef struct {
     int status;            /* MP_PLAN_* */
-    int num_waypoints;     /* waypoints written to `waypoints` */
+    int num_waypoints;     /* waypoints written to `waypoints` (MP_PLAN_PATH_TOO_LONG: the
+                              number the path needs; nothing is written) */
     float* waypoints;      /* caller buffer [capacity][dof], host memory */
     int capacity;
     int iterations;        /* samples consumed */
@@ -308,9 +332,14 @@ typedef struct {
 
 /* A server thread owns the device; submit/poll are thread-safe. */
 int mp_server_create(const mp_server_params* params, mp_server** out);
+/* The scene is pinned from submit until the query finishes (or the server is destroyed):
+ * mp_scene_update and mp_scene_destroy return MP_E_STATE for it meanwhile. */
 int mp_server_submit(mp_server* s, int robot_index, const mp_scene* scene, const float* start, const float* goal,
                      uint64_t seed, int64_t* ticket);
-/* Returns 1 and fills `out` when the query has finished, 0 if still pending. */
+/* Returns 1 and fills `out` when the query has finished, 0 if still pending.  out->waypoints
+ * holds out->capacity waypoints of the ticket's robot (capacity x its dof floats); a solved path
+ * longer than that is reported as MP_PLAN_PATH_TOO_LONG with its length and not copied.  Each
+ * result is handed out once. */
 int mp_server_poll(mp_server* s, int64_t ticket, mp_path* out);
 int mp_server_wait_idle(mp_server* s, double timeout_s);
 int mp_server_stats_get(mp_server* s, mp_server_stats* out);

@@ -570,7 +570,11 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
             X.os << "            work += " << tcost << "u;\n";
             X.os << "            if (__ballot(" << cond << ")) {\n";
             X.os << "                work += " << n * (kXform + tcost) << "u;\n";
-            if (kind == 1 && !cost) X.os << "                bool band = false;\n";
+            // spheres and capsules: contact <=> d2 < (r+R)^2 and sqrtf(d2) < r+R (the cost
+            // (r+R) - sqrtf(d2) is then > 0); a fast test with a 1e-6 relative margin, then the
+            // exact test for lanes in the thin band below the threshold
+            const bool banded = (kind == 1 || kind == 2) && !cost;
+            if (banded) X.os << "                bool band = false;\n";
             std::vector<std::string> d2n, srn;
             for (int s : L.spheres) {
                 V c[3];
@@ -607,12 +611,16 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
                 } else if (kind == 2) {
                     V d2 = X.tmp("mp_cap_d2(" + X.ref(c[0]) + ", " + X.ref(c[1]) + ", " + X.ref(c[2]) + ", x1, y1, z1, xv, yv, zv, rdv)");
                     V rs = X.tmp(lit(r) + " + crad");
-                    V rr = X.tmp(X.ref(rs) + " * " + X.ref(rs));
-                    if (cost)
-                        X.os << "                part += " << X.ref(d2) << " < " << X.ref(rr) << " ? " << X.ref(rr) << " - "
-                             << X.ref(d2) << " : 0.0f;\n";
-                    else
-                        X.os << "                hit |= " << X.ref(d2) << " < " << X.ref(rr) << ";\n";
+                    if (cost) {
+                        X.os << "                part += " << X.ref(d2) << " < " << X.ref(rs) << " * " << X.ref(rs)
+                             << " ? fmaxf(0.0f, " << X.ref(rs) << " - __builtin_sqrtf(" << X.ref(d2) << ")) : 0.0f;\n";
+                    } else {
+                        V T2 = X.tmp(X.ref(rs) + " * " + X.ref(rs));
+                        X.os << "                hit |= " << X.ref(d2) << " < " << X.ref(T2) << " * 0.999999f;\n";
+                        X.os << "                band |= " << X.ref(d2) << " < " << X.ref(T2) << ";\n";
+                        d2n.push_back(X.ref(d2));
+                        srn.push_back(X.ref(rs));
+                    }
                 } else {
                     V l[3];
                     for (int a = 0; a < 3; a++) l[a] = X.row(mm[4 * a], c[0], mm[4 * a + 1], c[1], mm[4 * a + 2], c[2], mm[4 * a + 3]);
@@ -624,7 +632,7 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
                         X.os << "                hit |= " << X.ref(sdf) << " < " << lit(r) << ";\n";
                 }
             }
-            if (kind == 1 && !cost) {
+            if (banded) {
                 X.os << "                if (__ballot(band && !hit)) {\n";
                 for (size_t t = 0; t < d2n.size(); t++)
                     X.os << "                    hit |= (" << d2n[t] << " < " << srn[t] << " * " << srn[t] << ") && (__builtin_sqrtf("
@@ -696,16 +704,18 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
     o << "        return hit;\n    }\n";
     // full-evaluation costs in the oracle order: spheres, boxes, cylinders, capsules; culled
     // blocks contribute exact zeros, so skipping them leaves every partial sum unchanged
-    o << "    static __device__ __forceinline__ float env_cost(const float* F, const SceneDev& sc, unsigned& work) {\n";
-    o << "        float tot = 0.0f, part = 0.0f;\n" << group_prologue;
+    // (per kind into pk[0..3]: the per-constraint costs of the sphere, cuboid, cylinder and
+    // capsule constraints, States.h:205's [constraint][state] rows)
+    o << "    static __device__ __forceinline__ void env_cost(const float* F, const SceneDev& sc, unsigned& work, float* pk) {\n";
+    o << "        float part = 0.0f;\n" << group_prologue;
     o << "        for (int o = 0; o < mp_uniform(sc.ns); ++o) {\n" << sph_load << env_block(1, true) << "        }\n";
-    o << "        tot += part; part = 0.0f;\n";
+    o << "        pk[0] = part; part = 0.0f;\n";
     o << "        for (int o = 0; o < mp_uniform(sc.nb); ++o) {\n" << box_load << env_block(0, true) << "        }\n";
-    o << "        tot += part; part = 0.0f;\n";
+    o << "        pk[1] = part; part = 0.0f;\n";
     o << "        for (int o = 0; o < mp_uniform(sc.nc); ++o) {\n" << cyl_load << env_block(3, true) << "        }\n";
-    o << "        tot += part; part = 0.0f;\n";
+    o << "        pk[2] = part; part = 0.0f;\n";
     o << "        for (int o = 0; o < mp_uniform(sc.ncap); ++o) {\n" << cap_load << env_block(2, true) << "        }\n";
-    o << "        tot += part;\n        return tot;\n    }\n";
+    o << "        pk[3] = part;\n    }\n";
     // joint bounds: BoundaryConstraint.cu:56-92, tolerance 1e-4 on the squared sum
     o << "    static __device__ __forceinline__ float bounds_acc(const float* q) {\n        float acc = 0.0f;\n";
     for (int j = 0; j < m.dof; j++) {
@@ -770,8 +780,8 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
           << "(const float* q, long long n, SceneDev sc, unsigned char* v, float* c, unsigned long long* w) { mp_check_kernel<" << R
           << ">(q, n, sc, v, c, w); }\n";
         o << "extern \"C\" __global__ void __launch_bounds__(256) mp_cost_" << s
-          << "(const float* q, long long n, SceneDev sc, float* c, unsigned char* v, unsigned long long* w) { mp_cost_kernel<" << R
-          << ">(q, n, sc, c, v, w); }\n";
+          << "(const float* q, long long n, SceneDev sc, float* c, unsigned char* v, float* parts, unsigned long long* w) { mp_cost_kernel<" << R
+          << ">(q, n, sc, c, v, parts, w); }\n";
         o << "extern \"C\" __global__ void __launch_bounds__(256) mp_sample_" << s
           << "(u64 seed, u64 off, long long n, float* q) { mp_sample_kernel<" << R << ">(seed, off, n, q, MP_SAMPLER_UNIFORM); }\n";
         o << "extern \"C\" __global__ void __launch_bounds__(256) mp_halton_" << s

@@ -92,6 +92,11 @@ SceneDevHost scene_view(const mp_scene* sc) {
 RobotModel model_from_desc(const mp_robot_desc* d) {
     if (!d || d->num_links <= 0 || d->num_links > 64 || d->num_spheres < 0 || d->num_spheres > 1024)
         throw Error(MP_E_INVALID, "robot description: bad link/sphere counts");
+    if (!d->joint_types || !d->parents || !d->active || !d->joint_origins || !d->joint_axes || !d->lower || !d->upper ||
+        !d->defaults || !d->collision_enabled)
+        throw Error(MP_E_INVALID, "robot description: a per-link array is null");
+    if (d->num_spheres > 0 && (!d->sphere_link || !d->sphere_pos || !d->sphere_radius))
+        throw Error(MP_E_INVALID, "robot description: a per-sphere array is null");
     RobotModel m;
     m.name = d->name ? d->name : "robot";
     for (char& c : m.name)
@@ -521,6 +526,8 @@ int mp_scene_update(mp_scene* sc, const float* spheres, int ns, const float* box
         if (ns < 0 || nb < 0 || nc < 0 || ncap < 0) throw Error(MP_E_INVALID, "negative obstacle count");
         if (ns > sc->cap_s || nb > sc->cap_b || nc > sc->cap_c || ncap > sc->cap_k)
             throw Error(MP_E_CAPACITY, "the number of obstacles exceeds the scene's capacity");
+        if (sc->server_refs.load() > 0)
+            throw Error(MP_E_STATE, "the scene has server queries queued or running; update it after they finish");
         MP_HIP(hipStreamSynchronize(G.stream));
         scene_upload(sc, spheres, ns, boxes, nb, cylinders, nc, capsules, ncap);
     });
@@ -529,6 +536,8 @@ int mp_scene_update(mp_scene* sc, const float* spheres, int ns, const float* box
 int mp_scene_destroy(mp_scene* sc) {
     return guard([&] {
         if (!sc) return;
+        if (sc->server_refs.load() > 0)
+            throw Error(MP_E_STATE, "the scene has server queries queued or running; destroy it after they finish");
         if (sc->dev) {
             hipStreamSynchronize(G.stream);
             hipFree(sc->dev);
@@ -581,7 +590,21 @@ int mp_state_costs(const mp_robot* robot, const mp_scene* scene, const float* d_
         if (n == 0) return;
         long long nn = n;
         mp::SceneDevHost sv = mp::scene_view(scene);
-        void* args[] = {&d_q, &nn, &sv, &d_cost, &d_verdict, &G.work};
+        float* d_parts = nullptr;
+        void* args[] = {&d_q, &nn, &sv, &d_cost, &d_verdict, &d_parts, &G.work};
+        mp::launch(robot->module->fn("mp_cost_0"), (unsigned)((n + 255) / 256), 256, stream_of(stream), args, "cost");
+    });
+}
+
+int mp_constraint_costs(const mp_robot* robot, const mp_scene* scene, const float* d_q, int64_t n, float* d_parts,
+                        float* d_cost, uint8_t* d_verdict, void* stream) {
+    return guard([&] {
+        need_init();
+        if (!robot || !scene || n < 0 || (n && (!d_q || !d_parts))) throw Error(MP_E_INVALID, "bad arguments");
+        if (n == 0) return;
+        long long nn = n;
+        mp::SceneDevHost sv = mp::scene_view(scene);
+        void* args[] = {&d_q, &nn, &sv, &d_cost, &d_verdict, &d_parts, &G.work};
         mp::launch(robot->module->fn("mp_cost_0"), (unsigned)((n + 255) / 256), 256, stream_of(stream), args, "cost");
     });
 }

@@ -1,6 +1,7 @@
 // Internal declarations of the motion-planning library (host side).
 #pragma once
 
+#include <atomic>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -97,4 +98,7 @@ struct mp_scene {
     int cap_s = 0, cap_b = 0, cap_c = 0, cap_k = 0;
     float* dev = nullptr;  // one allocation: spheres | boxes | cylinders | capsules
     size_t offs[4] = {0, 0, 0, 0};
+    // queries of servers that are queued or active on this scene: while > 0 the scene's device
+    // memory is read by a server's stream, so update and destroy are refused
+    mutable std::atomic<int> server_refs{0};
 };

@@ -224,16 +224,21 @@ void harvest(const SlotPool& pool, int slot, int dof, const QueryStateHost& st, 
     out->device_ms = st.t_done >= st.t_init ? (double)(st.t_done - st.t_init) * 1e-5 : 0.0;  // 100 MHz clock
     out->num_waypoints = 0;
     if (st.status == MP_PLAN_SOLVED) {
-        if (st.path_len < 0) {
-            out->status = MP_PLAN_TREE_FULL;  // path longer than max_path
-            out->num_waypoints = -st.path_len;
+        if (st.path_len == -1) {  // a tree slot on the path never became readable: no path
+            out->status = MP_PLAN_PATH_LOST;
             return;
         }
-        int n = std::min(st.path_len, out->capacity);
-        if (n > 0 && out->waypoints)
-            std::memcpy(out->waypoints, pool.hpaths + (size_t)slot * pool.max_path * pool.dof_max,
-                        (size_t)n * dof * sizeof(float));
-        out->num_waypoints = n;
+        // the path needs more waypoints than the pool's max_path (path_len = -length) or the
+        // caller's buffer holds: report its length, copy nothing
+        const int need = st.path_len < 0 ? -st.path_len : st.path_len;
+        if (st.path_len < 0 || need > out->capacity || !out->waypoints) {
+            out->status = MP_PLAN_PATH_TOO_LONG;
+            out->num_waypoints = need;
+            return;
+        }
+        std::memcpy(out->waypoints, pool.hpaths + (size_t)slot * pool.max_path * pool.dof_max,
+                    (size_t)need * dof * sizeof(float));
+        out->num_waypoints = need;
     }
 }
 

@@ -46,7 +46,12 @@ struct ActiveQ {
     int64_t ticket;
     double t_submit;
     int dof;
+    const mp_scene* scene;
 };
+
+// A scene stays pinned (mp_scene_destroy / mp_scene_update refuse it) from the submit of a query
+// on it until that query is retired or the server is destroyed.
+void scene_release(const mp_scene* sc) { sc->server_refs.fetch_sub(1, std::memory_order_acq_rel); }
 
 }  // namespace
 
@@ -146,7 +151,7 @@ void mp_server::run() {
                     mp_robot* r = robots[p.robot];
                     pool->fill_query(slot, robot_slot[p.robot], r->model.dof, p.scene, p.start, p.goal, p.seed, prm.params.max_iterations,
                                      prm.params.sampler);
-                    active[slot] = ActiveQ{p.ticket, p.t_submit, r->model.dof};
+                    active[slot] = ActiveQ{p.ticket, p.t_submit, r->model.dof, p.scene};
                     fresh.push_back(slot);
                 }
                 list.clear();
@@ -191,6 +196,7 @@ void mp_server::run() {
                     if (fin[k].second.info.status == MP_PLAN_SOLVED) solved++;
                     t_last = t;
                     done.emplace(fin[k].first, std::move(fin[k].second));
+                    scene_release(active[slot].scene);
                     active.erase(slot);
                     free_slots.push_back(slot);
                 }
@@ -289,6 +295,7 @@ int mp_server_submit(mp_server* s, int robot_index, const mp_scene* scene, const
         {
             std::lock_guard<std::mutex> g(s->mu);
             if (!s->error.empty()) throw Error(MP_E_STATE, "server stopped: " + s->error);
+            scene->server_refs.fetch_add(1, std::memory_order_acq_rel);
             p.ticket = s->next_ticket++;
             s->queue.push_back(p);
             s->submitted++;
@@ -313,10 +320,14 @@ int mp_server_poll(mp_server* s, int64_t ticket, mp_path* out) {
         *out = d.info;
         out->waypoints = wp;
         out->capacity = cap;
-        int dof = d.info.num_waypoints > 0 ? (int)(d.waypoints.size() / d.info.num_waypoints) : 0;
-        int n = std::min(d.info.num_waypoints, cap);
-        if (wp && n > 0) std::memcpy(wp, d.waypoints.data(), (size_t)n * dof * sizeof(float));
-        out->num_waypoints = d.info.status == MP_PLAN_SOLVED ? n : d.info.num_waypoints;
+        // the buffer holds `capacity` waypoints of the ticket's robot; a longer path is reported
+        // with its length and not copied
+        const int n = d.info.num_waypoints;
+        if (d.info.status == MP_PLAN_SOLVED && (n > cap || !wp)) {
+            out->status = MP_PLAN_PATH_TOO_LONG;
+        } else if (d.info.status == MP_PLAN_SOLVED && n > 0) {
+            std::memcpy(wp, d.waypoints.data(), d.waypoints.size() * sizeof(float));
+        }
         s->done.erase(it);
         ready = 1;
     });
@@ -371,6 +382,8 @@ int mp_server_destroy(mp_server* s) {
         s->cv.notify_all();
         if (s->worker.joinable()) s->worker.join();
         if (s->stream) hipStreamSynchronize(s->stream);
+        for (const Pending& p : s->queue) scene_release(p.scene);
+        for (auto& kv : s->active) scene_release(kv.second.scene);
         if (s->graph) hipGraphExecDestroy(s->graph);
         if (s->graph_def) hipGraphDestroy(s->graph_def);
         s->pool.reset();

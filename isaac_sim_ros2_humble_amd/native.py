@@ -12,7 +12,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmpamd.so")
 
 MP_OK = 0
-PLAN_STATUS = {0: "running", 1: "solved", 2: "exhausted", 3: "tree_full", 4: "invalid"}
+MP_E_INVALID, MP_E_HIP, MP_E_RTC, MP_E_NOMEM, MP_E_CAPACITY, MP_E_STATE = -1, -2, -3, -4, -5, -6
+PLAN_STATUS = {0: "running", 1: "solved", 2: "exhausted", 3: "tree_full", 4: "invalid", 5: "path_lost",
+               6: "path_too_long"}
 
 
 class MPError(RuntimeError):
@@ -93,6 +95,7 @@ _SIGS = {
     "mp_fk_link_poses": (C.c_int, [P, P, C.c_int64, P, P]),
     "mp_check_states": (C.c_int, [P, P, P, C.c_int64, P, P, P]),
     "mp_state_costs": (C.c_int, [P, P, P, C.c_int64, P, P, P]),
+    "mp_constraint_costs": (C.c_int, [P, P, P, C.c_int64, P, P, P, P]),
     "mp_check_edges": (C.c_int, [P, P, P, P, C.c_int, C.c_float, P, P, P]),
     "mp_nn_argmin": (C.c_int, [P, C.c_int, C.c_int, C.c_int, P, C.c_int, P, P]),
     "mp_sample": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_int64, P, P]),
@@ -247,11 +250,11 @@ class Scene:
         self.h = h.value
 
     def update(self, scene):
-        self.scene = scene
-        self._keep = [scene.spheres, scene.boxes, scene.cylinders, scene.capsules]
         ns, nb, nc, ncap = scene.counts
         check(lib().mp_scene_update(self.h, _p(scene.spheres), ns, _p(scene.boxes), nb, _p(scene.cylinders), nc,
                                     _p(scene.capsules), ncap), "mp_scene_update")
+        self.scene = scene
+        self._keep = [scene.spheres, scene.boxes, scene.cylinders, scene.capsules]
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -325,6 +328,23 @@ def state_costs(robot, scene, q):
     dv = DeviceBuffer(n)
     check(lib().mp_state_costs(robot.h, scene.h, dq.ptr, n, dcost.ptr, dv.ptr, None), "mp_state_costs")
     return dcost.to_array(np.float32, (n,)), dv.to_array(np.uint8, (n,))
+
+
+COST_KINDS = ("env_sphere", "env_cuboid", "env_cylinder", "env_capsule", "self", "bounds")
+
+
+def constraint_costs(robot, scene, q):
+    """Per-constraint costs [6][n] (rows in COST_KINDS order), total cost [n], verdict [n]."""
+    q = np.ascontiguousarray(q, np.float32).reshape(-1, robot.dof)
+    n = len(q)
+    dq = DeviceBuffer.from_array(q)
+    dparts = DeviceBuffer(4 * n * len(COST_KINDS))
+    dcost = DeviceBuffer(4 * n)
+    dv = DeviceBuffer(n)
+    check(lib().mp_constraint_costs(robot.h, scene.h, dq.ptr, n, dparts.ptr, dcost.ptr, dv.ptr, None),
+          "mp_constraint_costs")
+    return (dparts.to_array(np.float32, (len(COST_KINDS), n)), dcost.to_array(np.float32, (n,)),
+            dv.to_array(np.uint8, (n,)))
 
 
 def check_edges(robot, scene, q0, q1, resolution):
@@ -540,21 +560,37 @@ class Server:
         check(lib().mp_server_create(C.byref(sp), C.byref(h)), "mp_server_create")
         self.h = h.value
         self.max_path = p.max_path
+        # ticket -> (scene, dof): the Scene object stays alive until its result is handed out
+        self._tickets = {}
 
     def submit(self, robot_index, scene, start, goal, seed=0):
         t = C.c_int64()
+        dof = self.robots[robot_index].dof
         s = np.ascontiguousarray(start, np.float32)
         g = np.ascontiguousarray(goal, np.float32)
+        if s.shape != (dof,) or g.shape != (dof,):
+            raise MPError("start/goal must have the robot's %d joints" % dof)
         check(lib().mp_server_submit(self.h, robot_index, scene.h, _p(s), _p(g), seed, C.byref(t)), "mp_server_submit")
+        self._tickets[t.value] = (scene, dof)
         return t.value
 
-    def poll(self, ticket, dof):
-        wp = np.zeros((self.max_path, dof), np.float32)
+    def poll(self, ticket, dof=None):
+        """The finished query's result (once), or None while it runs.  The waypoint buffer is
+        sized by the ticket's robot; `dof`, if given, must match it."""
+        if ticket not in self._tickets:
+            return None
+        tdof = self._tickets[ticket][1]
+        if dof is not None and dof != tdof:
+            raise MPError("ticket %d was submitted for a %d-joint robot, not %d" % (ticket, tdof, dof))
+        wp = np.zeros((self.max_path, tdof), np.float32)
         pth = Path(0, 0, _p(wp), self.max_path, 0, 0, 0, 0, 0.0, 0.0, 0.0)
         rc = lib().mp_server_poll(self.h, ticket, C.byref(pth))
         if rc < 0:
             check(rc, "mp_server_poll")
-        return _path_result(pth, wp, dof) if rc == 1 else None
+        if rc != 1:
+            return None
+        del self._tickets[ticket]
+        return _path_result(pth, wp, tdof)
 
     def wait_idle(self, timeout_s=60.0):
         check(lib().mp_server_wait_idle(self.h, timeout_s), "mp_server_wait_idle")
@@ -568,6 +604,7 @@ class Server:
         if getattr(self, "h", None) and _lib is not None:
             check(_lib.mp_server_destroy(self.h), "mp_server_destroy")
             self.h = None
+            self._tickets = {}
 
     def __del__(self):
         try:

@@ -238,3 +238,72 @@ def poisson_arrivals(rate, duration, seed=0):
         if t >= duration:
             return np.array(out)
         out.append(t)
+
+
+def sample_pairs(model, scene, rng, check, check_edge, n, batch=4096, max_rounds=16):
+    """Up to n collision-free (start, goal) pairs in one scene from batched checks, alternating
+    pairs whose straight segment is collision-free and blocked ones (either kind fills in when
+    the scene runs short of the other)."""
+    easy, hard = [], []
+    for _ in range(max_rounds):
+        q = uniform_configs(model, batch, rng)
+        ok = q[check(model, scene, q).astype(bool)]
+        m = len(ok) // 2
+        if m == 0:
+            continue
+        a, b = ok[:m], ok[m:2 * m]
+        free = check_edge(model, scene, a, b).astype(bool)
+        easy += list(zip(a[free], b[free]))
+        hard += list(zip(a[~free], b[~free]))
+        if len(easy) >= (n + 1) // 2 and len(hard) >= n // 2:
+            break
+    out = []
+    while len(out) < n and (easy or hard):
+        src = easy if (len(out) % 2 == 0 and easy) or not hard else hard
+        out.append(src.pop())
+    return out
+
+
+def c5_library(models, n_scenes, pairs_per_scene, check, check_edge, seed=0, batch=2048):
+    """Scenes for served planning: (robot index, scene, pairs), obstacle counts log-uniform in
+    [5, 200] over box / cylinder / sphere, each with a pool of collision-free start/goal pairs of
+    uneven difficulty."""
+    rng = np.random.default_rng(seed + 55)
+    lib = []
+    while len(lib) < n_scenes:
+        k = int(rng.integers(len(models)))
+        lo, hi, keep = workspace(models[k])
+        scene = random_scene(rng, log_uniform_count(rng), lo, hi, keep)
+        pairs = sample_pairs(models[k], scene, rng, check, check_edge, pairs_per_scene, batch=batch)
+        if len(pairs) >= max(2, pairs_per_scene // 2):
+            lib.append((k, scene, pairs))
+    return lib
+
+
+def c5_stream(lib, rate, seconds, p_reuse=0.8, resident_max=256, seed=0):
+    """Poisson arrivals for served planning over a scene library.  Each arrival is (t, scene,
+    pair, upload): with probability p_reuse the query goes to a scene already resident on the
+    device (uniformly), otherwise it brings the library's next scene, which is uploaded at its
+    arrival (upload = True); the resident set keeps the last resident_max scenes brought in.
+    Each query takes its scene's next start/goal pair (cycling through the pool)."""
+    rng = np.random.default_rng(seed + 9)
+    times = poisson_arrivals(rate, seconds, seed=seed)
+    resident, nxt, used = [], 0, {}
+    out = []
+    for t in times:
+        if resident and rng.random() < p_reuse:
+            s = resident[int(rng.integers(len(resident)))]
+            up = False
+        else:
+            s = nxt % len(lib)
+            nxt += 1
+            up = True
+            if s in resident:
+                resident.remove(s)
+            resident.append(s)
+            if len(resident) > resident_max:
+                resident.pop(0)
+        p = used.get(s, 0)
+        used[s] = p + 1
+        out.append((float(t), s, p % len(lib[s][2]), up))
+    return out

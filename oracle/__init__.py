@@ -46,6 +46,8 @@ def lib():
         L.orc_check_centres32.argtypes = [P, P, P, P, C.c_int64, P]
         L.orc_edges32.argtypes = [P, P, P, P, C.c_int, C.c_float, P, P]
         L.orc_nn.argtypes = [P, C.c_int, C.c_int, P]
+        L.orc_steer.argtypes = [P, P, C.c_int, C.c_int, C.c_float]
+        L.orc_constraint_costs.argtypes = [P, P, P, C.c_int64, P]
         L.orc_validate_path.argtypes = [P, P, P, C.c_int, C.c_float, P]
         L.orc_sample.argtypes = [P, C.c_uint64, C.c_uint64, P]
         L.orc_sample_halton.argtypes = [P, C.c_uint64, C.c_uint64, P]
@@ -159,6 +161,17 @@ def check(robot, scene, q, want_cost=False, want_centres=False, want_flops=False
     return v, cost, cen, fl
 
 
+COST_KINDS = ("env_sphere", "env_cuboid", "env_cylinder", "env_capsule", "self", "bounds")
+
+
+def constraint_costs(robot, scene, q):
+    """Per-constraint costs [6][n], rows in COST_KINDS order."""
+    q = np.ascontiguousarray(q, np.float32).reshape(-1, robot.dof)
+    parts = np.zeros((len(COST_KINDS), len(q)), np.float32)
+    lib().orc_constraint_costs(robot.h, scene.h, _p(q), len(q), _p(parts))
+    return parts
+
+
 def check_centres(robot, scene, q, centres):
     q = np.ascontiguousarray(q, np.float32).reshape(-1, robot.dof)
     c = np.ascontiguousarray(centres, np.float32)
@@ -180,6 +193,13 @@ def nn(tree, q):
     tree = np.ascontiguousarray(tree, np.float32)
     q = np.ascontiguousarray(q, np.float32)
     return lib().orc_nn(_p(tree), tree.shape[0], tree.shape[1], _p(q))
+
+
+def steer(frm, to, max_distance):
+    frm = np.ascontiguousarray(frm, np.float32)
+    out = np.array(to, np.float32, copy=True, order="C").reshape(frm.shape)
+    lib().orc_steer(_p(frm), _p(out), frm.shape[0], frm.shape[1], max_distance)
+    return out
 
 
 def validate_path(robot, scene, path, resolution=0.005):

@@ -498,8 +498,10 @@ static inline float cyl_term(const float* c, float r, const float* m) {
     return pen > 0.0f ? pen : 0.0f;
 }
 static inline float cap_term(const float* c, float r, const float* k) {
-    /* sphere_capsule (pRRTC/src/planning/utils.cuh:113-127): contact <=> |p - closest|^2 < (r+R)^2;
-     * the cost reported for a contact is (r+R)^2 - |p - closest|^2 */
+    /* sphere_capsule (pRRTC/src/planning/utils.cuh:113-127): the closest point of the segment
+     * p1 + t v, t = clamp((c - p1).v / |v|^2, 0, 1); contact <=> |c - closest|^2 < (r+R)^2.  The
+     * cost is a penetration depth, as for env spheres (EnvConstraintSphere.cu:89-93):
+     * max(0, (r+R) - |c - closest|) */
     float dot = dot3f(c[0] - k[0], k[3], c[1] - k[1], k[4], c[2] - k[2], k[5]);
     float cdf = fminf(fmaxf(dot * k[6], 0.0f), 1.0f);
     float px = fmaf(k[3], cdf, k[0]), py = fmaf(k[4], cdf, k[1]), pz = fmaf(k[5], cdf, k[2]);
@@ -507,7 +509,7 @@ static inline float cap_term(const float* c, float r, const float* k) {
     float d2 = dot3f(dx, dx, dy, dy, dz, dz);
     float rs = r + k[7];
     float rr = rs * rs;
-    return d2 < rr ? rr - d2 : 0.0f;
+    return d2 < rr ? fmaxf(0.0f, rs - sqrtf(d2)) : 0.0f;
 }
 
 static int bounds_ok(const orc_robot* r, const float* q, float* cost) {
@@ -523,27 +525,30 @@ static int bounds_ok(const orc_robot* r, const float* q, float* cost) {
     return 0;
 }
 
-/* Total cost in the fixed order env spheres, boxes, cylinders, capsules (obstacle-major,
- * then robot sphere), self pairs, bounds.  Returns 1 if valid (total == 0). */
-static int check_one_cost(const orc_robot* r, const orc_scene* sc, const float* q, const float* cen, float* total) {
-    float tot = 0.0f;
+/* Per-constraint costs in the fixed order env spheres, boxes, cylinders, capsules (each
+ * obstacle-major, then robot sphere), self pairs, bounds (ORC_COST_KINDS values), and their total
+ * summed in that order.  Returns 1 if valid (total == 0). */
+#define ORC_COST_KINDS 6
+static int check_one_cost(const orc_robot* r, const orc_scene* sc, const float* q, const float* cen, float* total,
+                          float* parts) {
+    float pk[ORC_COST_KINDS];
     int S = r->S;
     float part = 0.0f;
     for (int o = 0; o < sc->ns; o++)
         for (int s = 0; s < S; s++) part += sphere_term(cen + 3 * s, r->sphere_r[s], sc->sph + 4 * o);
-    tot += part;
+    pk[0] = part;
     part = 0.0f;
     for (int o = 0; o < sc->nb; o++)
         for (int s = 0; s < S; s++) part += box_term(cen + 3 * s, r->sphere_r[s], sc->box + 18 * o);
-    tot += part;
+    pk[1] = part;
     part = 0.0f;
     for (int o = 0; o < sc->nc; o++)
         for (int s = 0; s < S; s++) part += cyl_term(cen + 3 * s, r->sphere_r[s], sc->cyl + 14 * o);
-    tot += part;
+    pk[2] = part;
     part = 0.0f;
     for (int o = 0; o < sc->ncap; o++)
         for (int s = 0; s < S; s++) part += cap_term(cen + 3 * s, r->sphere_r[s], sc->cap + 8 * o);
-    tot += part;
+    pk[3] = part;
     part = 0.0f;
     for (int p = 0; p < r->P; p++) {
         const float* a = cen + 3 * r->pi_[p];
@@ -551,12 +556,24 @@ static int check_one_cost(const orc_robot* r, const orc_scene* sc, const float* 
         float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
         part += self_term(dot3f(dx, dx, dy, dy, dz, dz), r->pthr[p], r->psr[p]);
     }
-    tot += part;
-    float bc;
-    bounds_ok(r, q, &bc);
-    tot += bc;
+    pk[4] = part;
+    bounds_ok(r, q, &pk[5]);
+    float tot = 0.0f;
+    for (int k = 0; k < ORC_COST_KINDS; k++) tot += pk[k];
+    if (parts) memcpy(parts, pk, sizeof(pk));
     *total = tot;
     return tot == 0.0f;
+}
+
+void orc_constraint_costs(const orc_robot* r, const orc_scene* s, const float* q, int64_t n, float* parts) {
+    float* cen = (float*)malloc(sizeof(float) * 3 * (r->S + 1));
+    float pk[ORC_COST_KINDS], tot;
+    for (int64_t i = 0; i < n; i++) {
+        orc_fk32(r, q + i * r->dof, cen);
+        check_one_cost(r, s, q + i * r->dof, cen, &tot, pk);
+        for (int k = 0; k < ORC_COST_KINDS; k++) parts[k * n + i] = pk[k];
+    }
+    free(cen);
 }
 
 /* Early-out verdict in canonical order; returns 1 if valid; *flops = canonical work. */
@@ -612,7 +629,7 @@ static void* check_worker(void* arg) {
         orc_fk32(r, q, cen);
         if (j->cost) {
             float tot;
-            int v = check_one_cost(r, j->s, q, cen, &tot);
+            int v = check_one_cost(r, j->s, q, cen, &tot, NULL);
             j->cost[i] = tot;
             if (j->verdict) j->verdict[i] = (uint8_t)v;
             if (j->canon) check_one_canon(r, j->s, q, cen, &j->canon[i]);
@@ -674,6 +691,23 @@ void orc_edges32(const orc_robot* r, const orc_scene* s, const float* q0, const 
         }
         ok[e] = (uint8_t)good;
         length[e] = d;
+    }
+}
+
+void orc_steer(const float* from, float* to, int n, int dof, float max_distance) {
+    /* interpolate_kernel (SingleArmSpace.cu:219-249): d = |to - from|; when d > max, to <- from +
+     * (max / d)(to - from).  FP32 order: d2 accumulated by fmaf over the joints in order,
+     * correctly rounded sqrtf and division, each coordinate fmaf(f, to - from, from). */
+    for (int i = 0; i < n; i++) {
+        const float* a = from + (int64_t)i * dof;
+        float* b = to + (int64_t)i * dof;
+        float d2 = 0.0f;
+        for (int j = 0; j < dof; j++) { float df = a[j] - b[j]; d2 = fmaf(df, df, d2); }
+        float d = sqrtf(d2);
+        if (d > max_distance) {
+            float f = max_distance / d;
+            for (int j = 0; j < dof; j++) b[j] = fmaf(f, b[j] - a[j], a[j]);
+        }
     }
 }
 

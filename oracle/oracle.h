@@ -21,6 +21,7 @@
  *                      capsule: pRRTC/src/planning/utils.cuh:113-127).
  *   - orc_edges32     edge interpolation check (SingleArmSpace.cu:646-848, KernelGenerator.h:408-522).
  *   - orc_nn          brute-force nearest neighbour (SingleArmStates.cu:1265-1431), ties -> lowest index.
+ *   - orc_steer       steering toward a target by at most a distance (SingleArmSpace.cu:219-249).
  *   - orc_validate_path  independent path re-check at a fine resolution.
  *   - orc_rrtconnect  CPU RRT-Connect with OMPL RRTConnect semantics (range, discrete motion
  *                     validation) over the FP32 checker: the CPU baseline.
@@ -72,6 +73,10 @@ void orc_link_poses32(const orc_robot* r, const float* q, float* poses);
  * stopping at the first hit).  nthreads >= 1 partitions configs over host threads. */
 void orc_check32(const orc_robot* r, const orc_scene* s, const float* q, int64_t n,
                  uint8_t* verdict, float* cost, float* centres, double* canon_flops, int nthreads);
+/* Per-constraint costs parts[6][n] (row k, configuration i at k*n + i): 0 env spheres, 1 boxes,
+ * 2 cylinders, 3 capsules, 4 self pairs, 5 joint bounds -- the [constraint][state] cost rows of
+ * BaseStates (States.h:205); orc_check32's total cost is their sum in this order. */
+void orc_constraint_costs(const orc_robot* r, const orc_scene* s, const float* q, int64_t n, float* parts);
 /* Verdicts from given sphere centres [n][S][3] plus q for the bound check (collision stage only). */
 void orc_check_centres32(const orc_robot* r, const orc_scene* s, const float* q, const float* centres,
                          int64_t n, uint8_t* verdict);
@@ -79,6 +84,9 @@ void orc_check_centres32(const orc_robot* r, const orc_scene* s, const float* q,
 /* Edge check with CUDAMPLib checkMotions semantics.  q0/q1 [m][dof]. */
 void orc_edges32(const orc_robot* r, const orc_scene* s, const float* q0, const float* q1, int m,
                  float resolution, uint8_t* ok, float* length);
+
+/* Steering (SingleArmSpace.cu:219-249): in place on to[n][dof]. */
+void orc_steer(const float* from, float* to, int n, int dof, float max_distance);
 
 /* nearest neighbour: tree [n][dof] row-major, query [dof]; returns index (ties -> lowest). */
 int orc_nn(const float* tree, int n, int dof, const float* q);

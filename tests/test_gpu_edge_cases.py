@@ -6,7 +6,8 @@ import pytest
 import oracle
 from isaac_sim_ros2_humble_amd import native, workloads
 from isaac_sim_ros2_humble_amd.scene import Scene, box_record
-from test_gpu_planner import _assert_valid, _robot, planner_states
+from helpers import assert_valid as _assert_valid, planner_states
+from test_gpu_planner import _robot
 
 pytestmark = pytest.mark.gpu
 

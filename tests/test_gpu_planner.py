@@ -1,12 +1,10 @@
 """GPU RRT-Connect (plain and batched): statuses, path validity re-checked by the CPU oracle at
 0.005 rad, endpoint identity, determinism per seed in exclusive mode, batch == solo."""
-import copy
-
 import numpy as np
 import pytest
 
 import oracle
-from helpers import point_robot
+from helpers import assert_valid as _assert_valid, point_robot
 from isaac_sim_ros2_humble_amd import native, workloads
 from isaac_sim_ros2_humble_amd.scene import Scene, box_record
 
@@ -29,41 +27,6 @@ def _robot(model):
 
 def gpu_edge(model, scene, a, b):
     ok, _ = native.check_edges(_robot(model), native.Scene(scene), a, b, 0.02)
-    return ok
-
-
-def planner_states(path, resolution):
-    """The states the planner checked along a returned path: each segment a->b is a tree edge
-    (states a + k * res * (b - a)/|b - a|, k = 0 .. |b - a|/res, endpoints included) or the join
-    between the trees (shorter than 2 res; only its endpoints are checked, CPRRTC semantics)."""
-    out = [path[:1].astype(np.float64)]
-    for a, b in zip(path[:-1].astype(np.float64), path[1:].astype(np.float64)):
-        # tree edges are whole multiples of res (up to float32 rounding, < 1e-5 relative); the
-        # join is shorter than 2 res and gets its endpoint only
-        n = max(int(np.floor(np.linalg.norm(b - a) / resolution + 1e-4)), 1)
-        out.append(a + (b - a) * (np.arange(1, n + 1)[:, None] / n))
-    return np.concatenate(out).astype(np.float32)
-
-
-def _assert_valid(model, scene, res, start, goal, resolution=0.02):
-    """Status, endpoints and the planner's contract (every state it checked along the path is
-    free); returns the re-validation at 0.005 rad, which a path may fail where it grazes an
-    obstacle between two checked states (a rate, not a per-path guarantee)."""
-    assert res["status"] == "solved", res
-    path = res["path"]
-    assert len(path) >= 2
-    assert np.array_equal(path[0], np.asarray(start, np.float32))
-    assert np.array_equal(path[-1], np.asarray(goal, np.float32))
-    orc, osc = oracle.OracleRobot(model), oracle.OracleScene(scene)
-    # planner_states rebuilds the checked states in float64 from the waypoints; each lies within
-    # ~1e-6 m (sphere centres) of the float32 state the planner checked, so a state on a contact
-    # boundary can flip: the contract is checked with spheres 1e-5 m smaller.  A wrong tree link
-    # shows up as a run of states deep in collision, which this still catches.
-    shrunk = copy.deepcopy(model)
-    shrunk.sphere_radius = (np.asarray(model.sphere_radius, np.float64) - 1e-5).astype(np.float32)
-    v, _, _, _ = oracle.check(oracle.OracleRobot(shrunk), osc, planner_states(path, resolution))
-    assert v.all(), "path states in collision: %d of %d" % ((v == 0).sum(), len(v))
-    ok, bad = oracle.validate_path(orc, osc, path, 0.005)
     return ok
 
 

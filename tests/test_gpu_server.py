@@ -78,3 +78,38 @@ def test_served_shared_mode_results_valid(fr3, fetch, lanes, options):
     assert nvalid >= 0.9 * nsol
     assert srv.poll(tickets[0], 7) is None  # each result is handed out once
     srv.close()
+
+
+def test_scene_pinned_while_queries_use_it(fr3):
+    """A scene with queued or running server queries cannot be updated or destroyed (MP_E_STATE);
+    the Python server keeps the Scene of each pending ticket alive, so a scene handle the caller
+    drops at once stays valid until its result is polled."""
+    import oracle
+    from helpers import point_robot
+    from isaac_sim_ros2_humble_amd.scene import Scene, box_record
+    m = point_robot(0.02, lim=1.0)
+    walls = [box_record([0.5 + dx, dy, dz], np.eye(3), h) for dx, dy, dz, h in [
+        (-0.15, 0, 0, [0.01, 0.2, 0.2]), (0.15, 0, 0, [0.01, 0.2, 0.2]), (0, -0.15, 0, [0.2, 0.01, 0.2]),
+        (0, 0.15, 0, [0.2, 0.01, 0.2]), (0, 0, -0.15, [0.2, 0.2, 0.01]), (0, 0, 0.15, [0.2, 0.2, 0.01])]]
+    shell = Scene(boxes=np.array(walls))
+    srv = native.Server([robot(m)], tree_memory_bytes=256 << 20, tree_capacity=65536, max_active=1,
+                        params=native.default_params(max_iterations=400000))
+    s, g = np.array([-0.5, 0, 0], np.float32), np.array([0.5, 0, 0], np.float32)
+    blocker = native.Scene(shell)
+    t1 = srv.submit(0, blocker, s, g, seed=1)          # unreachable goal: holds the only slot
+    free_scene = Scene(boxes=box_record([0, 0.8, 0], np.eye(3), [0.05, 0.05, 0.05])[None])
+    pinned = native.Scene(free_scene)
+    t2 = srv.submit(0, pinned, s, g, seed=2)           # queued behind it
+    L = native.lib()
+    assert L.mp_scene_destroy(pinned.h) == native.MP_E_STATE
+    assert L.mp_scene_update(pinned.h, None, 0, None, 0, None, 0, None, 0) == native.MP_E_STATE
+    t3 = srv.submit(0, native.Scene(free_scene), s, g, seed=3)  # the caller keeps no handle
+    srv.wait_idle(120.0)
+    r1, r2, r3 = srv.poll(t1), srv.poll(t2), srv.poll(t3)
+    assert r1["status"] in ("exhausted", "tree_full")
+    for r in (r2, r3):
+        assert r["status"] == "solved"
+        ok, _ = oracle.validate_path(oracle.OracleRobot(m), oracle.OracleScene(free_scene), r["path"], 0.005)
+        assert ok
+    pinned.update(free_scene)                          # released once its query finished
+    srv.close()

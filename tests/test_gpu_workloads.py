@@ -1,0 +1,164 @@
+"""The configured planning workloads on the GPU at their real shapes, every solved path checked by
+the oracle (the planner's contract at the planning resolution, and re-validation at 0.005 rad,
+whose rate must be at least VALIDITY_MIN):
+
+  C3  Fetch 8-DOF in a 3-shelf unit plus 30 boxes, start outside, goal between shelves; 32 seeds.
+  C4  256 mixed FR3 / Fetch queries (5-200 obstacles, half straight-line-free) in one launch with
+      2^16-node trees and the autotuned shape, against 256 plain one-query launches; in exclusive
+      mode the batch equals the plain launches bit for bit.
+  C5  a Poisson stream at 1000 queries/s for 2 s: scenes reused with probability 0.8, otherwise
+      uploaded while the server runs; a fresh start/goal per query; p99 latency under 50 ms; a
+      sample of queries solved alone gives the same statuses (shared mode is not deterministic,
+      so "the same" is the status, with both paths valid; exclusive mode is bit-exact, tested in
+      test_gpu_server.py).
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from helpers import assert_valid, assert_valid_many
+from isaac_sim_ros2_humble_amd import native, workloads
+
+pytestmark = pytest.mark.gpu
+
+VALIDITY_MIN = 0.9
+THREADS = min(16, os.cpu_count() or 1)
+
+_R = {}
+
+
+def robot(m):
+    if m.name not in _R:
+        native.init(0)
+        _R[m.name] = native.Robot(m)
+    return _R[m.name]
+
+
+def _check(m, scene, q):
+    return native.check_states(robot(m), native.Scene(scene), q)
+
+
+def _edge(m, scene, a, b):
+    return native.check_edges(robot(m), native.Scene(scene), a, b, 0.02)[0]
+
+
+def test_c3_shelf_seeds(fetch):
+    scene, shelf = workloads.c3_scene(seed=0)
+    ns = native.Scene(scene)
+    starts, goals = workloads.c3_queries(fetch, scene, shelf, 32, _check,
+                                         lambda m, q: native.fk_spheres(robot(m), q), seed=0)
+    items = []
+    for k in range(32):
+        r = native.rrtc_solve(robot(fetch), ns, starts[k], goals[k], native.default_params(seed=k))
+        assert r["status"] in ("solved", "exhausted"), r["status"]
+        if r["status"] == "solved":
+            items.append((fetch, scene, r, starts[k], goals[k]))
+    print("c3: %d/32 solved" % len(items))
+    assert len(items) >= 30
+    valid = assert_valid_many(items, THREADS)
+    assert valid >= VALIDITY_MIN * len(items), (valid, len(items))
+
+
+@pytest.fixture(scope="module")
+def c4(fr3, fetch):
+    models = [fr3, fetch]
+    qs = workloads.c4_batch(models, 256, _check, _edge, seed=0)
+    return models, qs
+
+
+def _c4_args(models, qs):
+    return ([robot(models[k]) for k, _, _, _, _ in qs], [native.Scene(sc) for _, sc, _, _, _ in qs],
+            [s for _, _, s, _, _ in qs], [g for _, _, _, g, _ in qs])
+
+
+def test_c4_batch_of_256_against_plain_launches(c4):
+    """The headline shape: 256 queries in one persistent launch, 2^16-node trees, launch shape
+    from the autotuner (calibrated on a separate 64-query sample), and the same queries as 256
+    plain launches.  Both solve almost all queries and keep the path-validity bar."""
+    models, qs = c4
+    robots, scenes, starts, goals = _c4_args(models, qs)
+    cal = workloads.c4_batch(models, 64, _check, _edge, seed=1)
+    cr, cs, cst, cg = _c4_args(models, cal)
+    best, cands = native.rrtc_autotune(cr, cs, cst, cg, native.default_params(seed=77), reps=1)
+    assert len(cands) >= 30
+    p = native.default_params(seed=0, tree_capacity=65536)  # launch-shape fields 0: the tuned shape
+    batch = native.rrtc_solve_batch(robots, scenes, starts, goals, p)
+    plain = [native.rrtc_solve(robots[i], scenes[i], starts[i], goals[i], native.default_params(seed=i))
+             for i in range(len(qs))]
+    for name, res in (("batch", batch), ("plain", plain)):
+        assert all(r["status"] in ("solved", "exhausted") for r in res), name
+        items = [(models[q[0]], q[1], r, q[2], q[3]) for q, r in zip(qs, res) if r["status"] == "solved"]
+        valid = assert_valid_many(items, THREADS)
+        print("c4 %s: %d/256 solved, %d valid at 0.005 rad" % (name, len(items), valid))
+        assert len(items) >= 0.97 * len(qs), name
+        assert valid >= VALIDITY_MIN * len(items), name
+    # the easy half (straight-line-free pairs) is solved by both
+    easy = [i for i, q in enumerate(qs) if q[4]]
+    assert all(batch[i]["status"] == "solved" and plain[i]["status"] == "solved" for i in easy)
+
+
+def test_c4_exclusive_batch_equals_plain_launches(c4):
+    """Exclusive mode (one team per query, deterministic per seed): the 256-query launch returns
+    exactly the paths of the 256 plain launches (query k with seed + k)."""
+    models, qs = c4
+    robots, scenes, starts, goals = _c4_args(models, qs)
+    req = native.BatchRequest(robots, scenes, starts, goals)
+    p = native.default_params(exclusive=1, seed=500, tree_capacity=65536)
+    a = req.solve(p)
+    pa = [a.path(k) for k in range(req.nq)]
+    sa, ia = list(a.status), a.iterations.copy()
+    b = req.solve_each(p)
+    assert sa == list(b.status)
+    assert np.array_equal(ia, b.iterations)
+    for k in range(req.nq):
+        assert np.array_equal(pa[k], b.path(k)), k
+    assert sum(s == "solved" for s in sa) >= 0.97 * req.nq
+
+
+def test_c5_poisson_stream_with_scene_uploads(fr3, fetch):
+    models = [fr3, fetch]
+    lib = workloads.c5_library(models, 48, 48, _check, _edge, seed=3)
+    stream = workloads.c5_stream(lib, 1000.0, 2.0, p_reuse=0.8, resident_max=32, seed=4)
+    srv = native.Server([robot(fr3), robot(fetch)], tree_memory_bytes=8 << 30, tree_capacity=65536,
+                        max_active=1024, use_graph=True, params=native.default_params())
+    dev, tickets, uploads = {}, [], 0
+    t0 = time.perf_counter()
+    for t, s, p, up in stream:
+        while time.perf_counter() - t0 < t:
+            time.sleep(0.0002)
+        if up or s not in dev:
+            dev.pop(s, None)
+            dev[s] = native.Scene(lib[s][1])  # uploaded while the server plans
+            uploads += 1
+        a, b = lib[s][2][p]
+        tickets.append((srv.submit(lib[s][0], dev[s], a, b, seed=len(tickets)), s, p))
+        if up and len(dev) > 40:  # drop the caller's handle of an old scene; the server keeps
+            dev.pop(next(iter(dev)))  # the scenes of its pending tickets alive
+    srv.wait_idle(120.0)
+    st = srv.stats()
+    res = [srv.poll(tk) for tk, _, _ in tickets]
+    srv.close()
+    n = len(tickets)
+    assert all(r is not None for r in res)
+    assert st["completed"] == n and st["submitted"] == n
+    assert 0.1 < uploads / n < 0.3
+    items = [(models[lib[s][0]], lib[s][1], r, *lib[s][2][p]) for (tk, s, p), r in zip(tickets, res)
+             if r["status"] == "solved"]
+    valid = assert_valid_many(items, THREADS)
+    print("c5: %d queries, %d uploads, %d solved, %d valid, p50 %.2f ms p99 %.2f ms, %.0f q/s" % (
+        n, uploads, len(items), valid, st["p50_ms"], st["p99_ms"], st["throughput_qps"]))
+    assert len(items) >= 0.97 * n
+    assert valid >= VALIDITY_MIN * len(items)
+    assert st["p99_ms"] < 50.0
+    # alone vs in the mix: 40 queries of the stream solved alone, with the served parameters
+    diff = 0
+    for i in range(0, n, max(1, n // 40))[:40]:
+        s, p = tickets[i][1], tickets[i][2]
+        a, b = lib[s][2][p]
+        alone = native.rrtc_solve(robot(models[lib[s][0]]), native.Scene(lib[s][1]), a, b, native.default_params(seed=i))
+        diff += alone["status"] != res[i]["status"]
+        if alone["status"] == "solved":
+            assert_valid(models[lib[s][0]], lib[s][1], alone, a, b)
+    assert diff <= 1

@@ -115,3 +115,37 @@ def test_ctypes_structs_match_the_header(tmp_path):
         got = [int(x) for x in line.split()]
         assert got[0] == C.sizeof(py), c
         assert got[1:] == [getattr(py, f).offset for f, _ in py._fields_], c
+
+
+def _gen_error(d):
+    arr = (C.POINTER(native.RobotDesc) * 1)(C.pointer(d))
+    n = native.lib().mp_generate_source(C.cast(arr, C.c_void_p), 1, None, 0)
+    assert n == 0
+    return native.lib().mp_last_error().decode()
+
+
+def test_malformed_robot_descriptions_rejected():
+    """Bad counts and null arrays are MP_E_INVALID errors, not crashes (no device needed)."""
+    m = load_robot("fr3")
+    for field, value, msg in (("num_links", 0, "link/sphere counts"), ("num_links", -3, "link/sphere counts"),
+                              ("num_spheres", -1, "link/sphere counts"), ("joint_types", None, "null"),
+                              ("parents", None, "null"), ("collision_enabled", None, "null"),
+                              ("sphere_radius", None, "null")):
+        d, keep = native.robot_desc(m)
+        setattr(d, field, value)
+        assert msg in _gen_error(d), field
+    # no spheres at all is a valid (collision-free) robot
+    d, keep = native.robot_desc(m)
+    d.num_spheres = 0
+    d.sphere_link = d.sphere_pos = d.sphere_radius = None
+    arr = (C.POINTER(native.RobotDesc) * 1)(C.pointer(d))
+    assert native.lib().mp_generate_source(C.cast(arr, C.c_void_p), 1, None, 0) > 0
+
+
+def test_plan_statuses_named():
+    # every MP_PLAN_* code of the header has a name in the Python layer
+    txt = open(HEADER).read()
+    codes = {int(v): k for k, v in re.findall(r"#define (MP_PLAN_[A-Z_]+) (\d+)\b", txt) if k != "MP_PLAN_PROFILE_N"}
+    assert set(codes) == set(native.PLAN_STATUS)
+    for v, k in codes.items():
+        assert native.PLAN_STATUS[v] == k[len("MP_PLAN_"):].lower()

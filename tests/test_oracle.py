@@ -349,3 +349,193 @@ def test_rrtconnect_rejects_invalid_start():
     so = oracle.OracleScene(Scene(boxes=box_record([0, 0, 0], np.eye(3), [0.05, 0.6, 0.6])[None]))
     path, st = oracle.rrtconnect(ro, so, np.zeros(3, np.float32), np.array([0.5, 0, 0], np.float32))
     assert st.solved == -1 and len(path) == 0
+
+
+# ------------------------------------------------------------------------------ cost magnitudes
+# Closed forms in float64, within 1e-6 m: a sphere of radius r at a known distance from each
+# obstacle kind.  A wrong cost formula (a squared distance, a dropped radius, a wrong sign, a
+# term booked in the wrong constraint row) fails these; the verdict-only checks above would not.
+COST_TOL = 1e-6
+
+
+def _parts(scene, pts, radius, base_sphere=None):
+    """Per-constraint rows [6][n] and the total, which must be the rows summed in order."""
+    m = point_robot(radius, base_sphere=base_sphere)
+    r, s = oracle.OracleRobot(m), oracle.OracleScene(scene)
+    q = np.asarray(pts, np.float32)
+    v, cost, _, _ = oracle.check(r, s, q, want_cost=True)
+    parts = oracle.constraint_costs(r, s, q)
+    tot = np.zeros(len(q), np.float32)
+    for k in range(len(oracle.COST_KINDS)):
+        tot = (tot + parts[k]).astype(np.float32)
+    assert np.array_equal(tot, cost)
+    assert np.array_equal(v == 1, cost == 0)
+    return parts, cost
+
+
+def _only_row(parts, k):
+    others = np.delete(parts, k, axis=0)
+    assert np.all(others == 0), others
+    return parts[k].astype(np.float64)
+
+
+def test_sphere_obstacle_cost_closed_form(rng):
+    r, R, o = 0.05, 0.2, np.array([0.1, -0.2, 0.3])
+    u = rng.normal(size=(5, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    depth = np.array([r / 3, r / 2, r, 1.5 * r, 0.0]) + np.array([0, 0, 0, 0, -0.01])
+    pts = o + u * (R + r - depth)[:, None]           # distance R + r - depth from the centre
+    parts, _ = _parts(Scene(spheres=np.array([[*o, R]], np.float32)), pts, r)
+    got = _only_row(parts, 0)
+    assert np.allclose(got, np.maximum(depth, 0), atol=COST_TOL)
+
+
+def test_box_cost_closed_form(rng):
+    r = 0.04
+    Rm = random_rotation(rng)
+    pos, half = np.array([0.2, 0.1, 0.5]), np.array([0.15, 0.05, 0.1])
+    e = np.ones(3) / np.sqrt(3.0)
+    local = np.array([[half[0] + r / 2, 0.0, 0.0],          # face, distance r/2: cost r/2
+                      [0.0, -(half[1] + r / 4), 0.01],      # face, distance r/4: cost 3r/4
+                      [0.01, 0.01, -0.02],                  # inside, distance 0: cost r
+                      list(half + 0.6 * r * e),             # corner, distance 0.6r: cost 0.4r
+                      [half[0] + 1.2 * r, 0.0, 0.0]])       # clear: 0
+    want = np.array([r / 2, 0.75 * r, r, 0.4 * r, 0.0])
+    parts, _ = _parts(Scene(boxes=box_record(pos, Rm, half)[None]), pos + local @ Rm.T, r)
+    assert np.allclose(_only_row(parts, 1), want, atol=COST_TOL)
+
+
+def test_cylinder_cost_closed_form(rng):
+    r, R, h = 0.03, 0.08, 0.5
+    Rm = random_rotation(rng)
+    pos = np.array([0.0, 0.3, 0.4])
+    a = 0.5 * r / np.sqrt(2.0)
+    local = np.array([[R + r / 3, 0.0, 0.0],                # side at mid-height, sdf r/3: cost 2r/3
+                      [0.0, -(R + r / 3), 0.1],
+                      [0.0, 0.0, h / 2 + r / 4],            # cap, sdf r/4: cost 3r/4
+                      [0.0, 0.0, 0.0],                      # on the axis, sdf -min(R, h/2): cost r + R
+                      [R + a, 0.0, -(h / 2 + a)],           # rim, sdf r/2: cost r/2
+                      [R + 1.1 * r, 0.0, 0.0]])             # clear: 0
+    want = np.array([2 * r / 3, 2 * r / 3, 0.75 * r, r + R, 0.5 * r, 0.0])
+    parts, _ = _parts(Scene(cylinders=cylinder_record(pos, Rm, R, h)[None]), pos + local @ Rm.T, r)
+    assert np.allclose(_only_row(parts, 2), want, atol=COST_TOL)
+
+
+def test_capsule_cost_is_penetration_depth(rng):
+    r, R, ln = 0.05, 0.06, 0.4
+    Rm = random_rotation(rng)
+    pos = np.array([0.2, 0.0, 0.3])
+    ax, side = Rm[:, 2], Rm[:, 0]
+    pts = np.array([pos + side * (R + r / 2),               # beside the segment, d = R + r/2: cost r/2
+                    pos + ax * 0.1 - side * (R + r / 4),    # cost 3r/4
+                    pos + ax * (ln / 2 + R + r / 4),        # beyond the end cap: distance to p2
+                    pos,                                    # on the segment: cost r + R
+                    pos + side * (R + 1.1 * r)])            # clear
+    want = np.array([r / 2, 0.75 * r, 0.75 * r, r + R, 0.0])
+    parts, _ = _parts(Scene(capsules=capsule_record(pos, Rm, R, ln)[None]), pts, r)
+    assert np.allclose(_only_row(parts, 3), want, atol=COST_TOL)
+
+
+def test_self_pair_cost_closed_form(rng):
+    # a second sphere fixed on the base link: the pair (moving, base) at distance d costs
+    # (ri + rj) - d when d < ri + rj
+    ri, rj, c = 0.05, 0.07, np.array([0.1, 0.0, 0.2])
+    u = rng.normal(size=(4, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    d = np.array([0.08, 0.0, 0.119, 0.2])
+    parts, _ = _parts(Scene(), c + u * d[:, None], ri, base_sphere=(*c, rj))
+    assert np.allclose(_only_row(parts, 4), np.maximum(ri + rj - d, 0), atol=COST_TOL)
+
+
+def test_joint_bound_cost_closed_form():
+    m = point_robot(0.01, lim=1.0)
+    r, s = oracle.OracleRobot(m), oracle.OracleScene(Scene())
+    q = np.array([[1.03, -1.04, 0.0], [1.005, 0.0, -1.005], [0.3, 0.2, 0.1]], np.float32)
+    parts = oracle.constraint_costs(r, s, q)
+    # sqrt of the summed squared violations; zero when that sum is below 1e-4
+    want = [np.hypot(0.03, 0.04), 0.0, 0.0]
+    assert np.allclose(_only_row(parts, 5), want, atol=COST_TOL)
+
+
+def test_costs_add_across_constraints():
+    # one configuration touching a sphere, a box and the base sphere at once: each row holds its
+    # own term and the total is their sum
+    r = 0.05
+    lo, hi = np.array([0.1, 0.13, -0.05]), np.array([0.2, 0.37, 0.05])
+    scene = Scene(spheres=np.array([[0.25, 0.1, 0.0, 0.08]], np.float32),
+                  boxes=box_record((lo + hi) / 2, np.eye(3), (hi - lo) / 2)[None])
+    p = np.array([[0.15, 0.1, 0.0]])
+    parts, cost = _parts(scene, p, r, base_sphere=(0.0, 0.0, 0.0, 0.16))
+    want = [0.13 - 0.1, r - 0.03, 0.0, 0.0, 0.21 - np.linalg.norm(p[0]), 0.0]
+    assert min(want[0], want[1], want[4]) > 0.01
+    assert np.allclose(parts[:, 0], want, atol=COST_TOL)
+    assert cost[0] == pytest.approx(sum(want), abs=4 * COST_TOL)
+
+
+# -------------------------------------------------------------------------------------- steering
+def test_steer_geometry(rng):
+    a = rng.normal(size=(400, 7)).astype(np.float32)
+    b = (a + rng.normal(size=(400, 7)) * rng.uniform(0.01, 0.5, size=(400, 1))).astype(np.float32)
+    out = oracle.steer(a, b, 0.3)
+    a64, b64, o64 = a.astype(np.float64), b.astype(np.float64), out.astype(np.float64)
+    d = np.linalg.norm(b64 - a64, axis=1)
+    near = d <= 0.3 * (1 - 1e-6)
+    far = d > 0.3 * (1 + 1e-6)
+    assert near.sum() > 50 and far.sum() > 50
+    assert np.array_equal(out[near], b[near])                      # within reach: the target itself
+    step = o64[far] - a64[far]
+    assert np.allclose(np.linalg.norm(step, axis=1), 0.3, rtol=1e-5)  # length = max
+    cosang = (step * (b64[far] - a64[far])).sum(1) / (np.linalg.norm(step, axis=1) * d[far])
+    assert np.all(cosang > 1 - 1e-6)                               # toward the target, not away
+
+
+# --------------------------------------------------------------------------------- RRT-Connect
+def test_rrtconnect_empty_scene_closed_form():
+    """Free space: the first iteration always connects.  The start tree extends toward sample 0
+    by at most `range` (node x), then the goal tree grows toward x in whole `range` steps until x
+    is within reach and is added.  The path is start, x, then the goal tree's nodes back to the
+    goal: every waypoint on the polyline start -> x -> goal, x -> goal cut into range pieces."""
+    m = point_robot(0.02, lim=1.0)
+    ro, so = oracle.OracleRobot(m), oracle.OracleScene(Scene())
+    rng_ = 0.3
+    for seed, (s, g) in enumerate([([-0.5, 0.0, 0.0], [0.5, 0.0, 0.0]), ([0.1, 0.2, 0.3], [0.2, 0.1, 0.3]),
+                                   ([-0.9, -0.9, -0.9], [0.9, 0.9, 0.9])]):
+        s, g = np.array(s, np.float32), np.array(g, np.float32)
+        path, st = oracle.rrtconnect(ro, so, s, g, range_=rng_, resolution=0.02, seed=seed, time_limit=10.0)
+        assert st.solved == 1 and st.iterations == 1
+        u = ro.sample(seed, 0, 1)[0].astype(np.float64)
+        s64, g64 = s.astype(np.float64), g.astype(np.float64)
+        x = s64 + (u - s64) * min(1.0, rng_ / np.linalg.norm(u - s64))
+        dxg = np.linalg.norm(x - g64)
+        k = int(np.ceil(dxg / rng_ - 1e-9))
+        # goal side from x: x, then g + (x - g) * j * range / |x - g| for j = k-1 .. 1, then g
+        goal_side = [g64 + (x - g64) * (j * rng_ / dxg) for j in range(k - 1, 0, -1)] + [g64]
+        want = np.array([s64, x] + goal_side)
+        assert path.shape == want.shape, (path.shape, want.shape)
+        assert np.allclose(path, want, atol=1e-5)
+
+
+def test_rrtconnect_path_invariants(rng):
+    """In clutter every returned path starts and ends at the query, no edge is longer than the
+    range, and every edge is free when re-checked at the planning resolution (spheres 1e-5 m
+    smaller, so that float64 re-interpolation cannot flip a state on a contact boundary)."""
+    m = point_robot(0.03, lim=1.0)
+    shrunk = point_robot(0.03 - 1e-5, lim=1.0)
+    ro, rs = oracle.OracleRobot(m), oracle.OracleRobot(shrunk)
+    boxes = [box_record(rng.uniform(-0.7, 0.7, 3), random_rotation(rng), rng.uniform(0.05, 0.2, 3)) for _ in range(12)]
+    scene = Scene(boxes=np.array(boxes))
+    so = oracle.OracleScene(scene)
+    solved = 0
+    for seed in range(12):
+        q = rng.uniform(-1, 1, size=(64, 3)).astype(np.float32)
+        v, _, _, _ = oracle.check(ro, so, q)
+        free = q[v == 1]
+        path, st = oracle.rrtconnect(ro, so, free[0], free[1], range_=0.3, resolution=0.02, seed=seed, time_limit=5.0)
+        if st.solved != 1:
+            continue
+        solved += 1
+        assert np.array_equal(path[0], free[0]) and np.array_equal(path[-1], free[1])
+        seg = np.linalg.norm(np.diff(path.astype(np.float64), axis=0), axis=1)
+        assert np.all(seg <= 0.3 * (1 + 1e-5)), seg.max()
+        assert oracle.validate_path(rs, so, path, 0.02)[0]
+    assert solved >= 10

@@ -59,3 +59,43 @@ def test_c3_shelf_and_queries(fetch):
 def test_poisson_arrivals_rate():
     t = workloads.poisson_arrivals(1000.0, 5.0, seed=0)
     assert abs(len(t) / 5.0 - 1000.0) < 100 and np.all(np.diff(t) > 0)
+
+
+def test_sample_pairs_alternate_difficulty(fr3):
+    check, edge = _orc([fr3])
+    rng = np.random.default_rng(3)
+    scene = workloads.random_scene(rng, 40, *workloads.workspace(fr3))
+    pairs = workloads.sample_pairs(fr3, scene, rng, check, edge, 8, batch=512)
+    assert len(pairs) == 8
+    a = np.array([p[0] for p in pairs])
+    b = np.array([p[1] for p in pairs])
+    assert check(fr3, scene, a).all() and check(fr3, scene, b).all()
+    free = edge(fr3, scene, a, b)
+    assert list(free[:6]) == [1, 0, 1, 0, 1, 0]
+
+
+def test_c5_library_pairs_are_free(fr3):
+    check, edge = _orc([fr3])
+    lib = workloads.c5_library([fr3], 3, 4, check, edge, seed=1, batch=256)
+    assert len(lib) == 3
+    for k, scene, pairs in lib:
+        assert k == 0 and 5 <= scene.num_obstacles <= 200 and len(pairs) >= 2
+        q = np.array([x for p in pairs for x in p])
+        assert check(fr3, scene, q).all()
+
+
+def test_c5_stream_reuse_and_uploads():
+    lib = [(0, None, [None] * (2 + i % 3)) for i in range(12)]
+    st = workloads.c5_stream(lib, 1000.0, 2.0, p_reuse=0.8, resident_max=8, seed=2)
+    assert abs(len(st) - 2000) < 200
+    up = np.array([a[3] for a in st])
+    assert abs(up.mean() - 0.2) < 0.04          # about one query in five brings a new scene
+    assert st[0][3]                              # the first query has nothing to reuse
+    # a reused scene was uploaded before and is among the last resident_max uploads
+    uploads = []
+    for t, s, p, u in st:
+        if u:
+            uploads.append(s)
+        else:
+            assert s in uploads[-8:]
+        assert 0 <= p < len(lib[s][2])
