@@ -238,10 +238,30 @@ typedef struct {
     float dd_radius;       /* rad (pRRTC 4.0) */
     float dd_alpha;        /* (pRRTC 1e-4) */
     float dd_min_radius;   /* rad (pRRTC 1.0) */
+    int balance;           /* which tree an iteration grows (MP_BALANCE_*) */
+    float tree_ratio;      /* the balance modes' size ratio (0: the mode's pRRTC default, 0.5 for
+                              MP_BALANCE_DYNAMIC, 1.0 for MP_BALANCE_ALTERNATE) */
 } mp_rrtc_params;
 
 #define MP_SAMPLER_UNIFORM 0
 #define MP_SAMPLER_HALTON 1
+
+/* Tree balance (pRRTC's balance settings, pRRTC/src/planning/pRRTC.cu:348-371; n_s, n_g the
+ * trees' sizes when an iteration starts, k its sample index, W the iterations running at once):
+ *   MP_BALANCE_SMALLER    grow the smaller tree (CPRRTC, RobotSolver.cu:570-588; the default)
+ *   MP_BALANCE_SPLIT      half the iterations grow each tree (pRRTC balance 0): k even -> start
+ *   MP_BALANCE_DYNAMIC    pRRTC balance 1: while |n_s - n_g| < 1.5 W, a fraction 1 - n_s/(n_s+n_g)
+ *                         of the iterations grows the start tree (chosen by a golden-ratio
+ *                         sequence over k); otherwise the start tree when n_s/(n_s+n_g) <
+ *                         tree_ratio, else the goal tree
+ *   MP_BALANCE_ALTERNATE  pRRTC balance 2 (a worker switches trees unless they are unbalanced),
+ *                         stateless: the smaller tree while the larger has at least (1 +
+ *                         tree_ratio) times its nodes, else alternate with k
+ * In paired iterations (lanes_per_iteration 32) both halves grow the tree chosen for the first. */
+#define MP_BALANCE_SMALLER 0
+#define MP_BALANCE_SPLIT 1
+#define MP_BALANCE_DYNAMIC 2
+#define MP_BALANCE_ALTERNATE 3
 
 /* Defaults: resolution 0.02, max_step 32, max_iterations 10000, tree_capacity 65536,
  * waves 0, max_path 4096, seed 0, exclusive 0, team_waves 0, teams_per_query 0,
@@ -307,6 +327,47 @@ typedef struct {
 int mp_rrtc_autotune(int nq, mp_robot* const* robots, const mp_scene* const* scenes, const float* starts16,
                      const float* goals16, const mp_rrtc_params* params, int per_query, int reps,
                      mp_tune_result* results, int cap, int* n_results, mp_rrtc_params* best);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Roadmap planner (RRG), device resident                                                   */
+/* ---------------------------------------------------------------------------------------- */
+/* The extend-connect loop of CUDAMPLib::RRG::solve (cudampl/CUDAMPLib/src/planners/RRG.cpp:
+ * 241-424) with the roadmap in device memory.  Vertices 0..ns-1 are the start states (group 1),
+ * ns..ns+ng-1 the goal states (group 2).  Each pass: sample k states from the space's Philox
+ * stream (pass p takes indices sample_offset + p k .. + k - 1, as mp_sample), pull each to within
+ * the side's travel distance of its nearest side-group member, check it, check the motions from
+ * each free sample to its nearest member of each group, then add the samples that reach a group
+ * (edges weighted by the motion length; a sample reaching both joins the groups and solves) or,
+ * when none reaches the side's group, halve its travel distance (floor 0.1) and stay on the
+ * side.  The side is group 1 on even loop indices.  No host round trip per pass: the caller
+ * checks its termination condition and calls mp_roadmap_iterate once per loop pass; the graph is
+ * read once at the end (Dijkstra stays with the caller, RRG.cpp:433-491).  The robot and scene
+ * must outlive the roadmap. */
+typedef struct mp_roadmap mp_roadmap;
+typedef struct {
+    int solved;            /* a vertex joins both groups */
+    int passes;            /* loop passes run */
+    int loop_index;        /* t: the next pass's side is group 1 when even */
+    int vertices, edges, start_group, goal_group;
+    float travel_start, travel_goal;  /* current travel distances of the two groups */
+} mp_roadmap_info;
+/* samples_per_iteration in [1, 256] (RRG::setSampleAttemptsInEachIteration); starts [ns][dof],
+ * goals [ng][dof] on the host. */
+int mp_roadmap_create(mp_robot* robot, const mp_scene* scene, const float* starts, int ns, const float* goals, int ng,
+                      int samples_per_iteration, float max_travel_distance, float resolution, uint64_t seed,
+                      uint64_t sample_offset, mp_roadmap** out);
+/* Enqueues one loop pass (asynchronous; at most a few passes are in flight).  *solved = 1 once a
+ * finished pass has joined the groups; no pass is enqueued then. */
+int mp_roadmap_iterate(mp_roadmap* rm, int* solved);
+/* Waits for the enqueued passes and reports the loop state. */
+int mp_roadmap_info_get(mp_roadmap* rm, mp_roadmap_info* info);
+/* Waits, then copies the graph: vertices [nv][dof], groups [nv] (1, 2, or 0 for the vertex that
+ * joins both), edges [ne][2] (added vertex, the member it reaches), weights [ne], and the members
+ * of each group in order of addition, start_members [start_group], goal_members [goal_group]
+ * (vertex ids).  Any pointer may be NULL. */
+int mp_roadmap_read(mp_roadmap* rm, float* vertices, int* groups, int* edges, float* weights, int* start_members,
+                    int* goal_members);
+int mp_roadmap_destroy(mp_roadmap* rm);
 
 /* ---------------------------------------------------------------------------------------- */
 /* Serving: continuous batching                                                             */

@@ -15,7 +15,7 @@ REPO = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libmpamd.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-SOURCES = ["mp_capi.cpp", "codegen.cpp", "rtc.cpp", "plan.cpp", "server.cpp"]
+SOURCES = ["mp_capi.cpp", "codegen.cpp", "rtc.cpp", "plan.cpp", "server.cpp", "roadmap.cpp"]
 
 
 def embed_template():

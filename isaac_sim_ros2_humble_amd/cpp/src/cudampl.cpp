@@ -442,25 +442,32 @@ void SingleArmStateManager::interpolateToStates(const BaseStatesPtr& states, std
     space_->interpolate(from, states, max_distance);
 }
 
-// util.cpp:205-248
-std::vector<std::vector<float>> interpolateVectors(const std::vector<float>& v1, const std::vector<float>& v2,
+// The densified straight segment a -> b (the segment densifier of
+// getUnconstrainedPathFromWaypoints, SingleArmSpace.cu:1428-1464 / util.cpp:205-248): a, then the
+// k = floor(|b - a| / resolution) points a + i (b - a) / k for i = 1 .. k (the last one b up to
+// rounding), then b itself, so both ends stay even where they repeat; just a when a == b.
+std::vector<std::vector<float>> interpolateVectors(const std::vector<float>& a, const std::vector<float>& b,
                                                    float resolution) {
-    if (v1.size() != v2.size()) throw std::runtime_error("Vectors must be of the same size");
-    if (v1 == v2) return {v1};
-    std::vector<std::vector<float>> out{v1};
-    float distance = 0.0f;
-    for (size_t i = 0; i < v1.size(); ++i) distance += std::pow(v2[i] - v1[i], 2);
-    distance = std::sqrt(distance);
-    int num_steps = (int)std::floor(distance / resolution);
-    std::vector<float> step(v1.size());
-    for (size_t i = 0; i < v1.size(); ++i) step[i] = (v2[i] - v1[i]) / num_steps;
-    for (int i = 1; i <= num_steps; ++i) {
-        std::vector<float> x(v1.size());
-        for (size_t j = 0; j < v1.size(); ++j) x[j] = v1[j] + i * step[j];
-        out.push_back(x);
+    if (a.size() != b.size()) throw std::runtime_error("interpolateVectors: the two states differ in dimension");
+    if (a == b) return {a};
+    const size_t n = a.size();
+    std::vector<float> inc(n);
+    float len2 = 0.0f;
+    for (size_t j = 0; j < n; ++j) {
+        const float d = b[j] - a[j];
+        len2 += d * d;
     }
-    out.push_back(v2);
-    return out;
+    const int k = (int)std::floor(std::sqrt(len2) / resolution);
+    for (size_t j = 0; j < n; ++j) inc[j] = k > 0 ? (b[j] - a[j]) / (float)k : 0.0f;
+    std::vector<std::vector<float>> pts;
+    pts.reserve((size_t)k + 2);
+    pts.push_back(a);
+    for (int i = 1; i <= k; ++i) {
+        pts.emplace_back(n);
+        for (size_t j = 0; j < n; ++j) pts.back()[j] = a[j] + (float)i * inc[j];
+    }
+    pts.push_back(b);
+    return pts;
 }
 
 // ---------------------------------------------------------------------------------- planners
@@ -540,58 +547,6 @@ SingleArmTask::SingleArmTask(const std::vector<std::vector<float>>& start_joint_
     goal_ = removeRedundantVectors(st->getJointStatesHost(), 1e-5f);
 }
 
-namespace {
-// The vertices of one RRG group as a structure-of-arrays device tree [dof][capacity], the layout
-// mp_nn_argmin scans.  A host mirror keeps the rows; appends upload one span per joint row and
-// growth (doubling) re-uploads the mirror.
-struct GroupTree {
-    size_t dof;
-    int n = 0, cap = 0;
-    std::vector<float> host;
-    std::unique_ptr<DevBuf> dev;
-    explicit GroupTree(size_t d) : dof(d) {}
-    void append(const std::vector<std::vector<float>>& qs) {
-        if (qs.empty()) return;
-        const int k = (int)qs.size(), need = n + k;
-        const bool grow = need > cap;
-        if (grow) {
-            int nc = std::max(64, cap);
-            while (nc < need) nc *= 2;
-            std::vector<float> h(dof * (size_t)nc, 0.0f);
-            for (size_t j = 0; j < dof; j++)
-                std::copy(host.begin() + j * cap, host.begin() + j * cap + n, h.begin() + j * nc);
-            host.swap(h);
-            cap = nc;
-            dev.reset(new DevBuf(host.size() * sizeof(float)));
-        }
-        for (int i = 0; i < k; i++)
-            for (size_t j = 0; j < dof; j++) host[j * cap + n + i] = qs[i][j];
-        if (grow) {
-            mpcpp::check(mp_memcpy_h2d(dev->p, host.data(), host.size() * sizeof(float), nullptr), "h2d");
-        } else {
-            for (size_t j = 0; j < dof; j++)
-                mpcpp::check(mp_memcpy_h2d((float*)dev->p + j * cap + n, host.data() + j * cap + n, k * sizeof(float),
-                                           nullptr),
-                             "h2d");
-        }
-        n = need;
-    }
-    // nearest vertex position in this group of each of the nq device states d_q[nq][dof]
-    void nearest(const float* d_q, int nq, int* d_idx, std::vector<int>& out) const {
-        mpcpp::check(mp_nn_argmin((const float*)dev->p, n, cap, (int)dof, d_q, nq, d_idx, nullptr), "mp_nn_argmin");
-        out.resize(nq);
-        mpcpp::check(mp_memcpy_d2h(out.data(), d_idx, nq * sizeof(int), nullptr), "d2h");
-    }
-};
-
-void upload_rows(void* d, const std::vector<std::vector<float>>& rows, size_t offset_rows, size_t dof) {
-    if (rows.empty()) return;
-    std::vector<float> flat;
-    flat.reserve(rows.size() * dof);
-    for (const auto& r : rows) flat.insert(flat.end(), r.begin(), r.end());
-    mpcpp::check(mp_memcpy_h2d((float*)d + offset_rows * dof, flat.data(), flat.size() * sizeof(float), nullptr), "h2d");
-}
-}  // namespace
 
 int RRG::add_vertex(const std::vector<float>& q) {
     node_.push_back(q);
@@ -723,98 +678,54 @@ void RRG::solve(BaseTerminationPtr termination) {
         }
     }
     termination->reset();
-    const int k = sample_attempts_ > 0 ? sample_attempts_ : 1;
-    float travel[3] = {0.0f, max_travel_distance_, max_travel_distance_};
-    GroupTree tree[3] = {GroupTree(dof), GroupTree(dof), GroupTree(dof)};
-    tree[1].append(states_of(members_[1]));
-    tree[2].append(states_of(members_[2]));
-    DevBuf d_smp((size_t)k * dof * sizeof(float)), d_from((size_t)k * dof * sizeof(float));
-    DevBuf d_q0((size_t)2 * k * dof * sizeof(float)), d_q1((size_t)2 * k * dof * sizeof(float));
-    DevBuf d_idx((size_t)2 * k * sizeof(int)), d_ok((size_t)2 * k), d_len((size_t)2 * k * sizeof(float));
-    std::vector<int> idx, idx_s, idx_g;
-    std::vector<uint8_t> ok;
-    std::vector<float> h_smp, len;
-    bool solved = false;
+    // RRG.cpp:241-424 on the device: one mp_roadmap pass per loop pass, the roadmap in device
+    // memory; the termination condition is checked once per pass, as in the reference's loop
+    const int k = std::min(std::max(sample_attempts_, 1), 256);
+    std::vector<float> sflat, gflat;
+    for (int v : S0) sflat.insert(sflat.end(), node_[v].begin(), node_[v].end());
+    for (int v : G0) gflat.insert(gflat.end(), node_[v].begin(), node_[v].end());
+    const unsigned long long offset = space->sampleStreamOffset();
+    mp_roadmap* rm = nullptr;
+    mpcpp::check(mp_roadmap_create(space->robot(), space->scene(), sflat.data(), (int)S0.size(), gflat.data(),
+                                   (int)G0.size(), k, max_travel_distance_, res, space->sampleStreamSeed(), offset, &rm),
+                 "mp_roadmap_create");
+    std::unique_ptr<mp_roadmap, int (*)(mp_roadmap*)> guard(rm, mp_roadmap_destroy);
+    bool terminated = false;
     for (int t = 0; t < 1000000; t++) {
         if (termination->checkTerminationCondition()) {
-            termination->printTerminationReason();
-            task_->setFailureReason("MeetTerminationCondition");
+            terminated = true;
             break;
         }
-        const int side = t % 2 == 0 ? 1 : 2;  // RRG.cpp:254-281: start group on even iterations
-        const std::vector<int> S = members_[1], G = members_[2];
-        const std::vector<int>& own = side == 1 ? S : G;
-        // sample, then pull each sample to within the side's travel distance of its nearest node
-        space->sampleInto((float*)d_smp.p, k);
-        tree[side].nearest((const float*)d_smp.p, k, (int*)d_idx.p, idx);
-        std::vector<std::vector<float>> from(k);
-        for (int i = 0; i < k; i++) from[i] = node_[own[idx[i]]];
-        upload_rows(d_from.p, from, 0, dof);
-        mpcpp::check(mp_steer((const float*)d_from.p, (float*)d_smp.p, k, (int)dof, travel[side], nullptr), "mp_steer");
-        mpcpp::check(mp_check_states(space->robot(), space->scene(), (const float*)d_smp.p, k, (uint8_t*)d_ok.p, nullptr,
-                                     nullptr),
-                     "mp_check_states");
-        ok.resize(k);
-        h_smp.resize((size_t)k * dof);
-        mpcpp::check(mp_memcpy_d2h(ok.data(), d_ok.p, k, nullptr), "d2h");
-        mpcpp::check(mp_memcpy_d2h(h_smp.data(), d_smp.p, h_smp.size() * sizeof(float), nullptr), "d2h");
-        std::vector<std::vector<float>> kept;
-        for (int i = 0; i < k; i++)
-            if (ok[i]) kept.emplace_back(h_smp.begin() + i * dof, h_smp.begin() + (i + 1) * dof);
-        if (kept.empty()) continue;
-        const int m = (int)kept.size();
-        // motions from each kept state to its nearest node of the start group, then of the goal group
-        upload_rows(d_q0.p, kept, 0, dof);
-        upload_rows(d_q0.p, kept, m, dof);
-        tree[1].nearest((const float*)d_q0.p, m, (int*)d_idx.p, idx_s);
-        tree[2].nearest((const float*)d_q0.p, m, (int*)d_idx.p, idx_g);
-        std::vector<std::vector<float>> to(2 * m);
-        for (int i = 0; i < m; i++) {
-            to[i] = node_[S[idx_s[i]]];
-            to[m + i] = node_[G[idx_g[i]]];
-        }
-        upload_rows(d_q1.p, to, 0, dof);
-        mpcpp::check(mp_check_edges(space->robot(), space->scene(), (const float*)d_q0.p, (const float*)d_q1.p, 2 * m,
-                                    res, (uint8_t*)d_ok.p, (float*)d_len.p, nullptr),
-                     "mp_check_edges");
-        ok.resize(2 * m);
-        len.resize(2 * m);
-        mpcpp::check(mp_memcpy_d2h(ok.data(), d_ok.p, 2 * m, nullptr), "d2h");
-        mpcpp::check(mp_memcpy_d2h(len.data(), d_len.p, 2 * m * sizeof(float), nullptr), "d2h");
-        int reached[3] = {0, 0, 0};
-        for (int i = 0; i < m; i++) {
-            reached[1] += ok[i] != 0;
-            reached[2] += ok[m + i] != 0;
-        }
-        // RRG.cpp:371-398: no state reached the side's group: halve its travel distance and
-        // stay on that side (the loop index skips one)
-        if (reached[side] == 0) {
-            const double half = travel[side] * 0.5;
-            travel[side] = half > 0.1 ? (float)half : (float)0.1;
-            t++;
-            continue;
-        }
-        travel[side] = max_travel_distance_;
-        std::vector<std::vector<float>> joined[3];
-        for (int i = 0; i < m; i++) {
-            const bool to_s = ok[i] != 0, to_g = ok[m + i] != 0;
-            if (!to_s && !to_g) continue;
-            const int v = add_vertex(kept[i]);
-            if (to_s) add_edge(v, S[idx_s[i]], len[i]);
-            if (to_g) add_edge(v, G[idx_g[i]], len[m + i]);
-            if (to_s && to_g) {
-                solved = true;  // the vertex joins the two groups
-            } else {
-                const int g = to_s ? 1 : 2;
-                members_[g].push_back(v);
-                joined[g].push_back(kept[i]);
-            }
-        }
-        tree[1].append(joined[1]);
-        tree[2].append(joined[2]);
+        int solved = 0;
+        mpcpp::check(mp_roadmap_iterate(rm, &solved), "mp_roadmap_iterate");
         if (solved) break;
     }
-    if (!solved) return;
+    mp_roadmap_info info;
+    mpcpp::check(mp_roadmap_info_get(rm, &info), "mp_roadmap_info_get");
+    space->advanceSampleStream((unsigned long long)info.passes * (unsigned long long)k);
+    // the device graph into the host one: device vertices 0..ns-1 are the start states, then the
+    // goal states, then the added states in order
+    std::vector<float> verts((size_t)info.vertices * dof);
+    std::vector<int> groups(info.vertices), edges(2 * (size_t)info.edges), smem(info.start_group), gmem(info.goal_group);
+    std::vector<float> weights(info.edges);
+    mpcpp::check(mp_roadmap_read(rm, verts.data(), groups.data(), edges.data(), weights.data(), smem.data(), gmem.data()),
+                 "mp_roadmap_read");
+    std::vector<int> host_of(info.vertices);
+    for (int v = 0; v < info.vertices; v++) {
+        if (v < (int)S0.size()) host_of[v] = S0[v];
+        else if (v < (int)(S0.size() + G0.size())) host_of[v] = G0[v - S0.size()];
+        else host_of[v] = add_vertex(std::vector<float>(verts.begin() + (size_t)v * dof, verts.begin() + (size_t)(v + 1) * dof));
+    }
+    for (int e = 0; e < info.edges; e++) add_edge(host_of[edges[2 * e]], host_of[edges[2 * e + 1]], weights[e]);
+    members_[1].clear();
+    members_[2].clear();
+    for (int v : smem) members_[1].push_back(host_of[v]);
+    for (int v : gmem) members_[2].push_back(host_of[v]);
+    if (!info.solved) {
+        if (terminated) termination->printTerminationReason();
+        task_->setFailureReason("MeetTerminationCondition");
+        return;
+    }
     std::vector<std::vector<float>> waypoints;
     for (int v : shortest_path())
         if (v != start_root_ && v != goal_root_) waypoints.push_back(node_[v]);

@@ -789,6 +789,14 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
         o << "extern \"C\" __global__ void __launch_bounds__(256) mp_edges_" << s
           << "(const float* a, const float* b, int m, float res, SceneDev sc, unsigned char* ok, float* len, unsigned long long* w) { mp_edges_kernel<"
           << R << ">(a, b, m, res, sc, ok, len, w); }\n";
+        if (robots.size() == 1) {  // the roadmap planner runs on a single robot's module
+            o << "extern \"C\" __global__ void __launch_bounds__(MP_RM_MAX_K) mp_rm_sample_" << s
+              << "(RoadmapDev g, SceneDev sc) { mp_rm_sample<" << R << ">(g, sc); }\n";
+            o << "extern \"C\" __global__ void __launch_bounds__(256) mp_rm_edges_" << s
+              << "(RoadmapDev g, SceneDev sc, unsigned long long* w) { mp_rm_edges<" << R << ">(g, sc, w); }\n";
+            o << "extern \"C\" __global__ void __launch_bounds__(64) mp_rm_commit_" << s << "(RoadmapDev g) { mp_rm_commit<"
+              << R << "::DOF>(g); }\n";
+        }
     }
     // planner: dispatch on the query's robot slot (wave-uniform)
     o << "__device__ __forceinline__ void mp_load_query(const QueryDev* p, QueryDev& Q) {\n"

@@ -56,6 +56,7 @@ hipStream_t lib_stream() { return G.stream; }
 int guard_call(const std::function<void()>& f) { return guard(f); }
 int lib_num_cus() { return G.num_cus; }
 unsigned long long* lib_plan_profile() { return G.prof_on ? G.prof : nullptr; }
+unsigned long long* lib_work_counter() { return G.work; }
 
 void launch(hipFunction_t f, unsigned grid, unsigned block, hipStream_t s, void** args, const char* family) {
     if (grid == 0) return;

@@ -59,11 +59,15 @@ struct PlanCtlHost {
     unsigned long long* prof;  // planner phase profile counters, null when off
     float dd_radius, dd_alpha, dd_min;  // dynamic-domain settings
     int dd;                             // 1: dynamic domain on
+    int balance;                        // MP_BALANCE_*
+    float tree_ratio;                   // the balance mode's ratio (resolved: never 0)
+    int window;                         // iterations running at once on one query (pRRTC's W)
 };
 
 hipStream_t lib_stream();
 int lib_num_cus();
 unsigned long long* lib_plan_profile();  // device counters when the phase profile is on, else null
+unsigned long long* lib_work_counter();  // the algorithmic-flop counter of the check kernels
 void launch(hipFunction_t f, unsigned grid, unsigned block, hipStream_t s, void** args, const char* family);
 SceneDevHost scene_view(const mp_scene* sc);
 void plan_release_workspaces();
@@ -97,11 +101,41 @@ struct SlotPool {
 
 size_t slot_bytes(int cap, int dof_max);  // device bytes of one slot
 void check_domain(const mp_rrtc_params& p);  // dynamic-domain settings (throws MP_E_INVALID)
+PlanCtlHost make_ctl(QueryDevHost* dq, QueryStateHost* dst, const int* dactive, int* dctl, const mp_rrtc_params& p);
+// Iterations running at once on one query for a launch shape (the balance modes' window).
+int balance_window(int teams, int cap, int lanes, bool exclusive);
 
 // Runs f, converting exceptions to an ABI status code (message in mp_last_error()).
 int guard_call(const std::function<void()>& f);
 void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params& p, int team, int teams,
                        int cap, int lanes, hipStream_t s);
+
+// The buffers one persistent solve uses: descriptors, states, active list and control block in
+// device memory, and their host staging (hctl holds 16 + nq ints; pinned or pageable).
+struct RunBuffers {
+    QueryDevHost* dq;
+    QueryStateHost* dst;
+    int* dactive;
+    int* dctl;
+    const QueryDevHost* hq;
+    int* hctl;
+    QueryStateHost* hst;
+};
+// Admission and planning launches for the nq queries of b.hq, then the states back into b.hst;
+// returns after the stream has synchronised.  family: the kernel-timing family of the planning
+// launch (null: untimed).
+void run_queries(Module& mod, const RunBuffers& b, int nq, const mp_rrtc_params& p, int team, int teams, int cap,
+                 int lanes, hipStream_t s, const char* family);
+
+// Module, module robot slots and launch shape of a solve (explicit params, else the tuned
+// shape of the robot set and class, else the defaults); validates the params.
+struct LaunchPlan {
+    std::shared_ptr<Module> mod;
+    std::vector<int> slot_of;
+    int dof_max = 1, team = 1, teams = 1, cap = 0, lanes = 64;
+};
+LaunchPlan plan_launch(int nq, mp_robot* const* robots, const mp_scene* const* scenes, const mp_rrtc_params* params,
+                       int default_wave_count);
 
 // Launch shape of the planning kernel: waves per team, total waves (0: resident teams) and
 // teams per query (0: unbounded).
@@ -116,6 +150,9 @@ Shape tuned_shape(const std::string& set_key, int cls);  // team 0 when nothing 
 void store_shape(const std::string& set_key, int cls, Shape sh);
 // Fills `out` from a finished query's state; call after the stream that ran the query has synchronised.
 void harvest(const SlotPool& pool, int slot, int dof, const QueryStateHost& st, mp_path* out);
+// The status and counters part of harvest; returns the number of path waypoints to copy into
+// out->waypoints (0: none).
+int harvest_status(const QueryStateHost& st, int max_iters, int cap, mp_path* out);
 constexpr int kDefaultTeamWaves = 4;
 // Iterations a team runs on the query its ticket picked (solve paths; the server uses 1).
 constexpr int kItersPerTicket = 4;

@@ -97,7 +97,9 @@ void SlotPool::fill_query(int slot, int robot_slot, int dof, const mp_scene* sce
     }
 }
 
-PlanCtlHost SlotPool::ctl(const mp_rrtc_params& p) const {
+PlanCtlHost SlotPool::ctl(const mp_rrtc_params& p) const { return make_ctl(dq, dst, dactive, dctl, p); }
+
+PlanCtlHost make_ctl(QueryDevHost* dq, QueryStateHost* dst, const int* dactive, int* dctl, const mp_rrtc_params& p) {
     PlanCtlHost c;
     c.q = dq;
     c.st = dst;
@@ -110,7 +112,15 @@ PlanCtlHost SlotPool::ctl(const mp_rrtc_params& p) const {
     c.dd_alpha = p.dd_alpha;
     c.dd_min = p.dd_min_radius;
     c.dd = p.dynamic_domain;
+    c.balance = p.balance;
+    c.tree_ratio = p.tree_ratio > 0.0f ? p.tree_ratio : (p.balance == MP_BALANCE_ALTERNATE ? 1.0f : 0.5f);
+    c.window = 1;  // set by the launcher from its shape
     return c;
+}
+
+int balance_window(int teams, int cap, int lanes, bool exclusive) {
+    const int per_query = exclusive ? 1 : (cap > 0 && cap < teams ? cap : teams);
+    return std::max(1, per_query * (lanes == 32 ? 2 : 1));
 }
 
 int default_teams(hipFunction_t plan_fn, int team_waves) {
@@ -124,6 +134,9 @@ int default_teams(hipFunction_t plan_fn, int team_waves) {
 int team_size(const mp_rrtc_params& p) { return p.team_waves > 0 ? p.team_waves : kDefaultTeamWaves; }
 
 void check_domain(const mp_rrtc_params& p) {
+    if (p.balance < MP_BALANCE_SMALLER || p.balance > MP_BALANCE_ALTERNATE)
+        throw Error(MP_E_INVALID, "balance must be one of the MP_BALANCE_* modes");
+    if (!(p.tree_ratio >= 0.0f) || p.tree_ratio > 1e6f) throw Error(MP_E_INVALID, "tree_ratio must be >= 0");
     if (p.dynamic_domain != 0 && p.dynamic_domain != 1) throw Error(MP_E_INVALID, "dynamic_domain must be 0 or 1");
     if (p.dynamic_domain && !(p.dd_radius > 0.0f && p.dd_min_radius > 0.0f && p.dd_alpha >= 0.0f && p.dd_alpha < 1.0f))
         throw Error(MP_E_INVALID, "dynamic domain needs dd_radius > 0, dd_min_radius > 0 and 0 <= dd_alpha < 1");
@@ -185,9 +198,15 @@ void plan_release_workspaces() {
 // of `teams` workgroups of `team` waves.
 void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params& p, int team, int teams,
                        int cap, int lanes, hipStream_t s) {
+    const RunBuffers b{pool.dq, pool.dst, pool.dactive, pool.dctl, pool.hq, pool.hpinned, pool.hst_pinned};
+    run_queries(mod, b, nq, p, team, teams, cap, lanes, s, "plan");
+}
+
+void run_queries(Module& mod, const RunBuffers& b, int nq, const mp_rrtc_params& p, int team, int teams, int cap,
+                 int lanes, hipStream_t s, const char* family) {
     const unsigned block = 64u * (unsigned)team;
-    MP_HIP(hipMemcpyAsync(pool.dq, pool.hq, (size_t)nq * sizeof(QueryDevHost), hipMemcpyHostToDevice, s));
-    int* h = pool.hpinned;
+    MP_HIP(hipMemcpyAsync(b.dq, b.hq, (size_t)nq * sizeof(QueryDevHost), hipMemcpyHostToDevice, s));
+    int* h = b.hctl;
     h[0] = 0;       // ticket
     h[1] = nq;      // running
     h[2] = nq;      // active-list length
@@ -204,21 +223,29 @@ void run_to_completion(Module& mod, SlotPool& pool, int nq, const mp_rrtc_params
     h[8] = cap;
     h[9] = pair_flag(lanes);
     for (int k = 0; k < nq; k++) h[16 + k] = k;
-    MP_HIP(hipMemcpyAsync(pool.dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, s));
-    MP_HIP(hipMemcpyAsync(pool.dactive, h + 16, (size_t)nq * sizeof(int), hipMemcpyHostToDevice, s));
-    PlanCtlHost ctl = pool.ctl(p);
+    MP_HIP(hipMemcpyAsync(b.dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, s));
+    MP_HIP(hipMemcpyAsync(b.dactive, h + 16, (size_t)nq * sizeof(int), hipMemcpyHostToDevice, s));
+    PlanCtlHost ctl = make_ctl(b.dq, b.dst, b.dactive, b.dctl, p);
+    ctl.window = balance_window(teams, cap, lanes, p.exclusive != 0);
     void* args[] = {&ctl};
-    launch(mod.fn("mp_plan_init"), (unsigned)nq, kInitBlock, s, args, "plan_init");
-    launch(mod.fn("mp_plan"), (unsigned)teams, block, s, args, "plan");
-    MP_HIP(hipMemcpyAsync(pool.hst_pinned, pool.dst, (size_t)nq * sizeof(QueryStateHost), hipMemcpyDeviceToHost, s));
+    launch(mod.fn("mp_plan_init"), (unsigned)nq, kInitBlock, s, args, family ? "plan_init" : nullptr);
+    launch(mod.fn("mp_plan"), (unsigned)teams, block, s, args, family);
+    MP_HIP(hipMemcpyAsync(b.hst, b.dst, (size_t)nq * sizeof(QueryStateHost), hipMemcpyDeviceToHost, s));
     MP_HIP(hipStreamSynchronize(s));
 }
 
 void harvest(const SlotPool& pool, int slot, int dof, const QueryStateHost& st, mp_path* out) {
+    const int n = harvest_status(st, pool.hq[slot].max_iters, pool.cap, out);
+    if (n > 0)
+        std::memcpy(out->waypoints, pool.hpaths + (size_t)slot * pool.max_path * pool.dof_max,
+                    (size_t)n * dof * sizeof(float));
+}
+
+int harvest_status(const QueryStateHost& st, int max_iters, int cap, mp_path* out) {
     out->status = st.status < 0 ? MP_PLAN_RUNNING : st.status;
-    out->iterations = std::min(st.sample_ctr, pool.hq[slot].max_iters);
-    out->start_tree_size = std::min(st.alloc[0], pool.cap);
-    out->goal_tree_size = std::min(st.alloc[1], pool.cap);
+    out->iterations = std::min(st.sample_ctr, max_iters);
+    out->start_tree_size = std::min(st.alloc[0], cap);
+    out->goal_tree_size = std::min(st.alloc[1], cap);
     out->configs_checked = (int64_t)st.checked;
     out->work_flops = (double)st.flops;
     out->device_ms = st.t_done >= st.t_init ? (double)(st.t_done - st.t_init) * 1e-5 : 0.0;  // 100 MHz clock
@@ -226,20 +253,19 @@ void harvest(const SlotPool& pool, int slot, int dof, const QueryStateHost& st, 
     if (st.status == MP_PLAN_SOLVED) {
         if (st.path_len == -1) {  // a tree slot on the path never became readable: no path
             out->status = MP_PLAN_PATH_LOST;
-            return;
+            return 0;
         }
         // the path needs more waypoints than the pool's max_path (path_len = -length) or the
         // caller's buffer holds: report its length, copy nothing
         const int need = st.path_len < 0 ? -st.path_len : st.path_len;
+        out->num_waypoints = need;
         if (st.path_len < 0 || need > out->capacity || !out->waypoints) {
             out->status = MP_PLAN_PATH_TOO_LONG;
-            out->num_waypoints = need;
-            return;
+            return 0;
         }
-        std::memcpy(out->waypoints, pool.hpaths + (size_t)slot * pool.max_path * pool.dof_max,
-                    (size_t)need * dof * sizeof(float));
-        out->num_waypoints = need;
+        return need;
     }
+    return 0;
 }
 
 }  // namespace mp
@@ -315,17 +341,18 @@ void store_shape(const std::string& set_key, int cls, Shape sh) {
     }
 }
 
-namespace {
-void solve_impl(const char* tag, int nq, mp_robot* const* robots, const mp_scene* const* scenes, const float* starts16,
-                const float* goals16, const mp_rrtc_params* params, mp_path* out, int default_wave_count) {
+LaunchPlan plan_launch(int nq, mp_robot* const* robots, const mp_scene* const* scenes, const mp_rrtc_params* params,
+                       int default_wave_count) {
     if (lib_stream() == nullptr) throw Error(MP_E_STATE, "mp_init() has not been called");
     check_params(params);
-    if (nq < 1 || !robots || !scenes || !starts16 || !goals16 || !out) throw Error(MP_E_INVALID, "bad batch arguments");
-    double t0 = now_ms();
+    if (nq < 1 || !robots || !scenes) throw Error(MP_E_INVALID, "bad batch arguments");
+    LaunchPlan lp;
     // robot set -> module (distinct robots, mapped to the module's canonical slots)
     std::vector<mp_robot*> uniq;
-    std::vector<int> slot_of(nq);
-    int dof_max = 1;
+    std::vector<int>& slot_of = lp.slot_of;
+    slot_of.assign(nq, 0);
+    int& dof_max = lp.dof_max;
+    dof_max = 1;
     for (int k = 0; k < nq; k++) {
         if (!robots[k] || !scenes[k]) throw Error(MP_E_INVALID, "null robot or scene in batch");
         int s = -1;
@@ -338,7 +365,7 @@ void solve_impl(const char* tag, int nq, mp_robot* const* robots, const mp_scene
         slot_of[k] = s;
         dof_max = std::max(dof_max, robots[k]->model.dof);
     }
-    std::shared_ptr<Module> mod;
+    std::shared_ptr<Module>& mod = lp.mod;
     if (uniq.size() == 1) {
         mod = uniq[0]->module;
     } else {
@@ -348,11 +375,6 @@ void solve_impl(const char* tag, int nq, mp_robot* const* robots, const mp_scene
         mod = get_module(ms, &module_slot);
         for (int k = 0; k < nq; k++) slot_of[k] = module_slot[slot_of[k]];
     }
-    hipStream_t s = lib_stream();
-    SlotPool& pool = pool_for(tag, nq, params->tree_capacity, dof_max, params->max_path);
-    for (int k = 0; k < nq; k++)
-        pool.fill_query(k, slot_of[k], robots[k]->model.dof, scenes[k], starts16 + 16 * k, goals16 + 16 * k,
-                        params->seed + (uint64_t)k, params->max_iterations, params->sampler);
     // launch shape: explicit parameters, else the tuned shape for this robot set and class, else
     // the built-in defaults
     int W = params->team_waves > 0 ? params->team_waves : kDefaultTeamWaves;
@@ -372,7 +394,25 @@ void solve_impl(const char* tag, int nq, mp_robot* const* robots, const mp_scene
     if (lanes == 0 || params->max_step > 32) lanes = params->max_step > 32 ? 64 : kDefaultLanes;
     int teams = waves > 0 ? std::max(1, waves / W) : default_teams(mod->fn("mp_plan"), W);
     if (params->exclusive) teams = nq;
-    run_to_completion(*mod, pool, nq, *params, W, teams, cap, lanes, s);
+    lp.team = W;
+    lp.teams = teams;
+    lp.cap = cap;
+    lp.lanes = lanes;
+    return lp;
+}
+
+namespace {
+void solve_impl(const char* tag, int nq, mp_robot* const* robots, const mp_scene* const* scenes, const float* starts16,
+                const float* goals16, const mp_rrtc_params* params, mp_path* out, int default_wave_count) {
+    if (!starts16 || !goals16 || !out) throw Error(MP_E_INVALID, "bad batch arguments");
+    double t0 = now_ms();
+    const LaunchPlan lp = plan_launch(nq, robots, scenes, params, default_wave_count);
+    hipStream_t s = lib_stream();
+    SlotPool& pool = pool_for(tag, nq, params->tree_capacity, lp.dof_max, params->max_path);
+    for (int k = 0; k < nq; k++)
+        pool.fill_query(k, lp.slot_of[k], robots[k]->model.dof, scenes[k], starts16 + 16 * k, goals16 + 16 * k,
+                        params->seed + (uint64_t)k, params->max_iterations, params->sampler);
+    run_to_completion(*lp.mod, pool, nq, *params, lp.team, lp.teams, lp.cap, lp.lanes, s);
     for (int k = 0; k < nq; k++) harvest(pool, k, robots[k]->model.dof, pool.hst_pinned[k], &out[k]);
     double lat = now_ms() - t0;
     for (int k = 0; k < nq; k++) out[k].latency_ms = lat;
@@ -400,6 +440,8 @@ void mp_rrtc_default_params(mp_rrtc_params* p) {
     p->dd_radius = 4.0f;
     p->dd_alpha = 1e-4f;
     p->dd_min_radius = 1.0f;
+    p->balance = MP_BALANCE_SMALLER;
+    p->tree_ratio = 0.0f;
 }
 
 int mp_rrtc_solve(mp_robot* robot, const mp_scene* scene, const float* start, const float* goal,
@@ -438,13 +480,19 @@ int mp_rrtc_autotune(int nq, mp_robot* const* robots, const mp_scene* const* sce
         std::vector<Cand> cands;
         std::vector<int> lane_opts{64};
         if (params->max_step <= 32) lane_opts.push_back(32);
+        // the resident grid of each team size; a 1024-wave candidate that is the resident grid
+        // too (one wave per SIMD on 256 CUs) is the same launch and is not run twice
+        std::shared_ptr<mp::Module> mod = mp::plan_launch(nq, robots, scenes, params, 0).mod;
         for (int l : lane_opts)
             for (int t : {1, 2, 4}) {
                 if (per_query) {
                     for (int w : {256, 512, 1024, 2048}) cands.push_back({t, w, 0, l});
                 } else {
-                    for (int w : {0, 1024})
+                    const bool same = mp::default_teams(mod->fn("mp_plan"), t) * t == 1024;
+                    for (int w : {0, 1024}) {
+                        if (w == 1024 && same) continue;
                         for (int c : {0, 16, 32, 64, 128}) cands.push_back({t, w, c, l});
+                    }
                 }
             }
         std::vector<mp_path> out((size_t)nq);

@@ -107,6 +107,7 @@ void mp_server::step(const std::vector<int>& list, int n_new) {
     if (prm.use_graph) {
         if (!graph) {
             mp::PlanCtlHost ctl = pool->ctl(prm.params);
+            ctl.window = mp::balance_window(teams, cap, lanes, prm.params.exclusive != 0);
             void* args[] = {&ctl};
             MP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             MP_HIP(hipMemcpyAsync(pool->dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, stream));
@@ -122,6 +123,7 @@ void mp_server::step(const std::vector<int>& list, int n_new) {
         MP_HIP(hipGraphLaunch(graph, stream));
     } else {
         mp::PlanCtlHost ctl = pool->ctl(prm.params);
+        ctl.window = mp::balance_window(teams, cap, lanes, prm.params.exclusive != 0);
         void* args[] = {&ctl};
         MP_HIP(hipMemcpyAsync(pool->dctl, h, 16 * sizeof(int), hipMemcpyHostToDevice, stream));
         MP_HIP(hipMemcpyAsync(pool->dactive, h + 16, list.size() * sizeof(int), hipMemcpyHostToDevice, stream));

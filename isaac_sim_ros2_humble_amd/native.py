@@ -34,7 +34,11 @@ class RRTCParams(C.Structure):
                 ("tree_capacity", C.c_int), ("waves", C.c_int), ("max_path", C.c_int), ("seed", C.c_uint64),
                 ("exclusive", C.c_int), ("team_waves", C.c_int), ("teams_per_query", C.c_int),
                 ("lanes_per_iteration", C.c_int), ("sampler", C.c_int), ("dynamic_domain", C.c_int),
-                ("dd_radius", C.c_float), ("dd_alpha", C.c_float), ("dd_min_radius", C.c_float)]
+                ("dd_radius", C.c_float), ("dd_alpha", C.c_float), ("dd_min_radius", C.c_float),
+                ("balance", C.c_int), ("tree_ratio", C.c_float)]
+
+
+BALANCE_SMALLER, BALANCE_SPLIT, BALANCE_DYNAMIC, BALANCE_ALTERNATE = 0, 1, 2, 3
 
 
 SAMPLER_UNIFORM, SAMPLER_HALTON = 0, 1
@@ -56,6 +60,12 @@ class ServerParams(C.Structure):
     _fields_ = [("num_robots", C.c_int), ("robots", C.c_void_p), ("tree_memory_bytes", C.c_size_t),
                 ("tree_capacity", C.c_int), ("max_active", C.c_int), ("step_iterations", C.c_int), ("waves", C.c_int),
                 ("use_graph", C.c_int), ("params", RRTCParams)]
+
+
+class RoadmapInfo(C.Structure):
+    _fields_ = [("solved", C.c_int), ("passes", C.c_int), ("loop_index", C.c_int), ("vertices", C.c_int),
+                ("edges", C.c_int), ("start_group", C.c_int), ("goal_group", C.c_int), ("travel_start", C.c_float),
+                ("travel_goal", C.c_float)]
 
 
 class ServerStats(C.Structure):
@@ -106,6 +116,12 @@ _SIGS = {
     "mp_rrtc_solve_batch": (C.c_int, [C.c_int, P, P, P, P, C.POINTER(RRTCParams), P]),
     "mp_rrtc_autotune": (C.c_int, [C.c_int, P, P, P, P, C.POINTER(RRTCParams), C.c_int, C.c_int, P, C.c_int,
                                    C.POINTER(C.c_int), C.POINTER(RRTCParams)]),
+    "mp_roadmap_create": (C.c_int, [P, P, P, C.c_int, P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, C.c_uint64,
+                                    C.POINTER(C.c_void_p)]),
+    "mp_roadmap_iterate": (C.c_int, [P, C.POINTER(C.c_int)]),
+    "mp_roadmap_info_get": (C.c_int, [P, C.POINTER(RoadmapInfo)]),
+    "mp_roadmap_read": (C.c_int, [P, P, P, P, P, P, P]),
+    "mp_roadmap_destroy": (C.c_int, [P]),
     "mp_server_create": (C.c_int, [C.POINTER(ServerParams), C.POINTER(C.c_void_p)]),
     "mp_server_submit": (C.c_int, [P, C.c_int, P, P, P, C.c_uint64, C.POINTER(C.c_int64)]),
     "mp_server_poll": (C.c_int, [P, C.c_int64, C.POINTER(Path)]),
@@ -542,6 +558,55 @@ def rrtc_autotune(robots, scenes, starts, goals, params=None, per_query=False, r
     return best, [{"team_waves": r.team_waves, "waves": r.waves, "teams_per_query": r.teams_per_query,
                    "lanes_per_iteration": r.lanes_per_iteration, "kernel_ms": r.kernel_ms}
                   for r in res[:min(n.value, 128)]]
+
+
+class Roadmap:
+    """Device-resident roadmap planner loop (mp_roadmap_*): the robot and scene outlive it."""
+
+    def __init__(self, robot, scene, starts, goals, samples_per_iteration=30, max_travel_distance=0.5,
+                 resolution=0.02, seed=0x5eed, sample_offset=0):
+        init()
+        self.robot, self.scene = robot, scene
+        s = np.ascontiguousarray(starts, np.float32).reshape(-1, robot.dof)
+        g = np.ascontiguousarray(goals, np.float32).reshape(-1, robot.dof)
+        h = C.c_void_p()
+        check(lib().mp_roadmap_create(robot.h, scene.h, _p(s), len(s), _p(g), len(g), samples_per_iteration,
+                                      max_travel_distance, resolution, seed, sample_offset, C.byref(h)),
+              "mp_roadmap_create")
+        self.h = h.value
+
+    def iterate(self):
+        """Enqueues one loop pass; True once a finished pass has joined the groups."""
+        s = C.c_int()
+        check(lib().mp_roadmap_iterate(self.h, C.byref(s)), "mp_roadmap_iterate")
+        return bool(s.value)
+
+    def info(self):
+        i = RoadmapInfo()
+        check(lib().mp_roadmap_info_get(self.h, C.byref(i)), "mp_roadmap_info_get")
+        return {f: getattr(i, f) for f, _ in RoadmapInfo._fields_}
+
+    def read(self):
+        inf = self.info()
+        nv, ne = inf["vertices"], inf["edges"]
+        inf["nv"], inf["ne"] = nv, ne  # counts; "vertices" and "edges" below are the arrays
+        v = np.zeros((nv, self.robot.dof), np.float32)
+        grp = np.zeros(nv, np.int32)
+        e = np.zeros((ne, 2), np.int32)
+        w = np.zeros(ne, np.float32)
+        sm = np.zeros(inf["start_group"], np.int32)
+        gm = np.zeros(inf["goal_group"], np.int32)
+        check(lib().mp_roadmap_read(self.h, _p(v), _p(grp), _p(e), _p(w), _p(sm), _p(gm)), "mp_roadmap_read")
+        inf.update(vertices=v, groups=grp, edges=e, weights=w, start_members=sm, goal_members=gm)
+        return inf
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.mp_roadmap_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Server:

@@ -48,6 +48,8 @@ def lib():
         L.orc_nn.argtypes = [P, C.c_int, C.c_int, P]
         L.orc_steer.argtypes = [P, P, C.c_int, C.c_int, C.c_float]
         L.orc_constraint_costs.argtypes = [P, P, P, C.c_int64, P]
+        L.orc_rrg.argtypes = [P, P, P, C.c_int, P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, C.c_uint64,
+                              C.c_int, P, P, C.c_int, P, P, C.c_int, P, P, P]
         L.orc_validate_path.argtypes = [P, P, P, C.c_int, C.c_float, P]
         L.orc_sample.argtypes = [P, C.c_uint64, C.c_uint64, P]
         L.orc_sample_halton.argtypes = [P, C.c_uint64, C.c_uint64, P]
@@ -228,6 +230,35 @@ def radical_inverse(i, b):
 
 def full_flops(robot, scene):
     return lib().orc_full_flops(robot.h, scene.h)
+
+
+class RrgInfo(C.Structure):
+    _fields_ = [("solved", C.c_int), ("passes", C.c_int), ("loop_index", C.c_int), ("vertices", C.c_int),
+                ("edges", C.c_int), ("start_group", C.c_int), ("goal_group", C.c_int), ("travel_start", C.c_float),
+                ("travel_goal", C.c_float)]
+
+
+def rrg(robot, scene, starts, goals, k, max_travel, resolution, seed, offset, max_passes, vcap=1 << 16):
+    """The roadmap loop for up to max_passes passes: a dict with the info fields and the graph."""
+    starts = np.ascontiguousarray(starts, np.float32).reshape(-1, robot.dof)
+    goals = np.ascontiguousarray(goals, np.float32).reshape(-1, robot.dof)
+    ecap = 2 * vcap
+    v = np.zeros((vcap, robot.dof), np.float32)
+    g = np.zeros(vcap, np.int32)
+    e = np.zeros((ecap, 2), np.int32)
+    w = np.zeros(ecap, np.float32)
+    sm = np.zeros(vcap, np.int32)
+    gm = np.zeros(vcap, np.int32)
+    info = RrgInfo()
+    rc = lib().orc_rrg(robot.h, scene.h, _p(starts), len(starts), _p(goals), len(goals), k, max_travel, resolution,
+                       seed, offset, max_passes, _p(v), _p(g), vcap, _p(e), _p(w), ecap, _p(sm), _p(gm), C.byref(info))
+    assert rc == 0, "orc_rrg: capacity"
+    out = {f: getattr(info, f) for f, _ in RrgInfo._fields_}
+    nv, ne = info.vertices, info.edges
+    out["nv"], out["ne"] = nv, ne  # counts; "vertices" and "edges" below are the arrays
+    out.update(vertices=v[:nv].copy(), groups=g[:nv].copy(), edges=e[:ne].copy(), weights=w[:ne].copy(),
+               start_members=sm[:info.start_group].copy(), goal_members=gm[:info.goal_group].copy())
+    return out
 
 
 def rrtconnect(robot, scene, start, goal, range_=0.3, resolution=0.02, seed=0, max_iterations=1_000_000,

@@ -722,6 +722,134 @@ int orc_nn(const float* tree, int n, int dof, const float* q) {
     return best;
 }
 
+/* Nearest member of q among members [cnt][dof] (a group's states in order of addition): squared
+ * L2 accumulated by fmaf over the joints, ties to the earlier member. */
+static int group_nearest(const float* members, int cnt, int dof, const float* q) {
+    int best = 0;
+    float bd = INFINITY;
+    for (int m = 0; m < cnt; m++) {
+        float acc = 0.0f;
+        for (int j = 0; j < dof; j++) { float d0 = q[j] - members[m * dof + j]; acc = fmaf(d0, d0, acc); }
+        if (acc < bd) { bd = acc; best = m; }
+    }
+    return best;
+}
+
+int orc_rrg(const orc_robot* r, const orc_scene* s, const float* starts, int ns, const float* goals, int ng, int k,
+            float max_travel, float resolution, uint64_t seed, uint64_t offset, int max_passes, float* vertices,
+            int* groups, int vcap, int* edges, float* weights, int ecap, int* start_members, int* goal_members,
+            orc_rrg_info* info) {
+    /* CUDAMPLib::RRG::solve's extend-connect loop (RRG.cpp:241-424), pass by pass. */
+    const int dof = r->dof;
+    if (ns < 1 || ng < 1 || k < 1 || ns + ng > vcap) return -1;
+    float* gq[2] = {(float*)malloc(sizeof(float) * dof * vcap), (float*)malloc(sizeof(float) * dof * vcap)};
+    int* gv[2] = {start_members, goal_members};
+    float* smp = (float*)malloc(sizeof(float) * dof * k);
+    float* from = (float*)malloc(sizeof(float) * dof * k);
+    float* q0 = (float*)malloc(sizeof(float) * dof * 2 * k);
+    float* q1 = (float*)malloc(sizeof(float) * dof * 2 * k);
+    uint8_t* freed = (uint8_t*)malloc(k);
+    uint8_t* ok = (uint8_t*)malloc(2 * k);
+    float* len = (float*)malloc(sizeof(float) * 2 * k);
+    int* nn1 = (int*)malloc(sizeof(int) * k);
+    int* nn2 = (int*)malloc(sizeof(int) * k);
+    int nv = ns + ng, ne = 0, cnt[2] = {ns, ng}, t = 0, pass = 0, solved = 0, rc = 0;
+    float travel[2] = {max_travel, max_travel};
+    /* vertices 0..ns-1: the start states (group 1), then the goal states (group 2) */
+    for (int v = 0; v < nv; v++) {
+        const float* q = v < ns ? starts + v * dof : goals + (v - ns) * dof;
+        memcpy(vertices + v * dof, q, sizeof(float) * dof);
+        groups[v] = v < ns ? 1 : 2;
+        int g = v < ns ? 0 : 1, m = v < ns ? v : v - ns;
+        memcpy(gq[g] + m * dof, q, sizeof(float) * dof);
+        gv[g][m] = v;
+    }
+    while (pass < max_passes && !solved) {
+        const int side = t & 1;
+        /* sample, pull to within the side's travel distance of the nearest side member, check */
+        int nfree = 0;
+        for (int i = 0; i < k; i++) {
+            orc_sample(r, seed, offset + (uint64_t)pass * (uint64_t)k + (uint64_t)i, smp + i * dof);
+            int m = group_nearest(gq[side], cnt[side], dof, smp + i * dof);
+            memcpy(from + i * dof, gq[side] + m * dof, sizeof(float) * dof);
+        }
+        orc_steer(from, smp, k, dof, travel[side]);
+        for (int i = 0; i < k; i++) {
+            freed[i] = (uint8_t)check_q(r, s, smp + i * dof);
+            nfree += freed[i];
+        }
+        if (nfree == 0) { /* RRG.cpp:300-306 */
+            t += 1;
+            pass++;
+            continue;
+        }
+        /* motions from each free sample to its nearest member of each group */
+        int m2 = 0;
+        for (int i = 0; i < k; i++) {
+            if (!freed[i]) continue;
+            nn1[i] = group_nearest(gq[0], cnt[0], dof, smp + i * dof);
+            nn2[i] = group_nearest(gq[1], cnt[1], dof, smp + i * dof);
+        }
+        for (int i = 0; i < k; i++) {
+            memcpy(q0 + i * dof, smp + i * dof, sizeof(float) * dof);
+            memcpy(q0 + (k + i) * dof, smp + i * dof, sizeof(float) * dof);
+            memcpy(q1 + i * dof, gq[0] + (freed[i] ? nn1[i] : 0) * dof, sizeof(float) * dof);
+            memcpy(q1 + (k + i) * dof, gq[1] + (freed[i] ? nn2[i] : 0) * dof, sizeof(float) * dof);
+        }
+        for (int e = 0; e < 2 * k; e++) {
+            int i = e < k ? e : e - k;
+            if (freed[i]) orc_edges32(r, s, q0 + e * dof, q1 + e * dof, 1, resolution, ok + e, len + e);
+            else { ok[e] = 0; len[e] = 0.0f; }
+        }
+        int reach[2] = {0, 0};
+        for (int i = 0; i < k; i++) {
+            reach[0] += freed[i] && ok[i];
+            reach[1] += freed[i] && ok[k + i];
+        }
+        (void)m2;
+        if (reach[side] == 0) { /* halve the side's travel distance (floor 0.1), same side */
+            double half = (double)travel[side] * 0.5;
+            travel[side] = half > 0.1 ? (float)half : 0.1f;
+            t += 2;
+            pass++;
+            continue;
+        }
+        if (nv + nfree > vcap || ne + 2 * nfree > ecap) { rc = -2; break; }
+        travel[side] = max_travel;
+        for (int i = 0; i < k; i++) {
+            int to1 = freed[i] && ok[i], to2 = freed[i] && ok[k + i];
+            if (!to1 && !to2) continue;
+            int v = nv++;
+            memcpy(vertices + v * dof, smp + i * dof, sizeof(float) * dof);
+            groups[v] = to1 && to2 ? 0 : (to1 ? 1 : 2);
+            if (to1) { edges[2 * ne] = v; edges[2 * ne + 1] = gv[0][nn1[i]]; weights[ne] = len[i]; ne++; }
+            if (to2) { edges[2 * ne] = v; edges[2 * ne + 1] = gv[1][nn2[i]]; weights[ne] = len[k + i]; ne++; }
+            if (to1 && to2) solved = 1;
+            else {
+                int g = to1 ? 0 : 1;
+                memcpy(gq[g] + cnt[g] * dof, smp + i * dof, sizeof(float) * dof);
+                gv[g][cnt[g]++] = v;
+            }
+        }
+        t += 1;
+        pass++;
+    }
+    if (info) {
+        info->solved = solved;
+        info->passes = pass;
+        info->loop_index = t;
+        info->vertices = nv;
+        info->edges = ne;
+        info->start_group = cnt[0];
+        info->goal_group = cnt[1];
+        info->travel_start = travel[0];
+        info->travel_goal = travel[1];
+    }
+    free(gq[0]); free(gq[1]); free(smp); free(from); free(q0); free(q1); free(freed); free(ok); free(len);
+    free(nn1); free(nn2);
+    return rc;
+}
+
 int orc_validate_path(const orc_robot* r, const orc_scene* s, const float* path, int k, float resolution,
                       int* bad_segment) {
     int dof = r->dof;

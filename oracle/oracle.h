@@ -108,6 +108,26 @@ float orc_radical_inverse(uint32_t i, uint32_t b);
 void orc_philox4x32(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
                     uint32_t out[4]);
 
+/* Roadmap planner: CUDAMPLib::RRG::solve's extend-connect loop (RRG.cpp:241-424) for up to
+ * max_passes loop passes or until a vertex joins the start and goal groups.  Pass p samples
+ * orc_sample(seed, offset + p k + i), i < k; each sample is pulled to within the side's travel
+ * distance of its nearest side-group member (orc_steer) and checked; motions (orc_edges32) go from
+ * each free sample to its nearest member of each group; the samples reaching a group are added in
+ * sample order with their edges (weight = motion length), or, when none reaches the side's group,
+ * its travel distance halves (floor 0.1) and the side repeats.  The side is group 1 on even loop
+ * indices.  Vertices 0..ns-1 are the starts (group 1), then the goals (group 2).  Outputs:
+ * vertices [vcap][dof], groups [vcap] (1, 2, or 0 joins both), edges [ecap][2] (added vertex,
+ * reached member), weights [ecap], start_members / goal_members [vcap] (vertex ids in order of
+ * addition).  Returns 0, or -2 when vcap / ecap is too small. */
+typedef struct {
+    int solved, passes, loop_index, vertices, edges, start_group, goal_group;
+    float travel_start, travel_goal;
+} orc_rrg_info;
+int orc_rrg(const orc_robot* r, const orc_scene* s, const float* starts, int ns, const float* goals, int ng, int k,
+            float max_travel, float resolution, uint64_t seed, uint64_t offset, int max_passes, float* vertices,
+            int* groups, int vcap, int* edges, float* weights, int ecap, int* start_members, int* goal_members,
+            orc_rrg_info* info);
+
 typedef struct {
     int solved;
     int iterations;

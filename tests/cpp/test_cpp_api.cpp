@@ -11,9 +11,20 @@
 #include <string>
 #include <vector>
 
+// the include block of src/foliation_planner/src/evaluate_cudampl.cpp:14-20, verbatim
+#include <CUDAMPLib/spaces/SingleArmSpace.h>
+#include <CUDAMPLib/constraints/EnvConstraintSphere.h>
+#include <CUDAMPLib/constraints/SelfCollisionConstraint.h>
+#include <CUDAMPLib/tasks/SingleArmTask.h>
+#include <CUDAMPLib/planners/RRG.h>
+#include <CUDAMPLib/termination/StepTermination.h>
+#include <CUDAMPLib/termination/TimeoutTermination.h>
+// and the rest of cuda_test.cpp:14-28 (its TaskSpaceConstraint.h is outside this library)
+#include <CUDAMPLib/constraints/EnvConstraintCuboid.h>
+#include <CUDAMPLib/constraints/EnvConstraintCylinder.h>
+#include <CUDAMPLib/constraints/BoundaryConstraint.h>
+#include <CUDAMPLib/planners/cRRTC.h>
 #include <CPRRTC/RobotSolver.h>
-#include <planners/RRG.h>
-#include <spaces/SingleArmSpace.h>
 #include "mp_cpp_util.h"
 
 using V = std::vector<double>;
@@ -39,9 +50,60 @@ static void print_path(const char* key, const std::vector<std::vector<float>>& p
     std::printf("\n");
 }
 
+// The RRG evaluation of src/foliation_planner/src/evaluate_cudampl.cpp:160-250 for each task
+// "task<k>_obstacles" (x y z per sphere), "task<k>_radius", "task<k>_start", "task<k>_goal":
+// self-collision plus a sphere-obstacle constraint named "obstacle_constraint" (replaced per
+// task), a 0.02-resolution space, RRG with max travel distance 6.0 and 60 samples per iteration,
+// a 10 s timeout, the path without densification.
+static int evaluate(std::map<std::string, V>& m, const std::vector<int>& jt, const std::vector<mpx::Isometry3d>& poses,
+                    const std::vector<mpx::Vector3d>& axes, const std::vector<int>& par, const std::vector<int>& smap,
+                    const std::vector<std::vector<float>>& spos, const std::vector<float>& rad,
+                    const std::vector<bool>& act, const std::vector<float>& lo, const std::vector<float>& hi,
+                    const std::vector<float>& def, const std::vector<std::string>& names,
+                    const std::vector<std::vector<bool>>& en, size_t dof) {
+    std::vector<CUDAMPLib::BaseConstraintPtr> constraints;
+    CUDAMPLib::SelfCollisionConstraintPtr self_collision_constraint =
+        std::make_shared<CUDAMPLib::SelfCollisionConstraint>("self_collision_constraint", smap, rad, en);
+    constraints.push_back(self_collision_constraint);
+    for (int i = 0; m.count("task" + std::to_string(i) + "_start"); i++) {
+        const std::string k = "task" + std::to_string(i) + "_";
+        for (size_t j = 0; j < constraints.size(); j++)
+            if (constraints[j]->getName() == "obstacle_constraint") {
+                constraints.erase(constraints.begin() + j);
+                break;
+            }
+        std::vector<std::vector<float>> obstacle_pos;
+        const V& ob = m[k + "obstacles"];
+        for (size_t o = 0; o + 2 < ob.size(); o += 3) obstacle_pos.push_back({(float)ob[o], (float)ob[o + 1], (float)ob[o + 2]});
+        std::vector<float> radius(m[k + "radius"].begin(), m[k + "radius"].end());
+        CUDAMPLib::EnvConstraintSpherePtr env_constraint =
+            std::make_shared<CUDAMPLib::EnvConstraintSphere>("obstacle_constraint", obstacle_pos, radius);
+        constraints.push_back(env_constraint);
+        CUDAMPLib::SingleArmSpacePtr single_arm_space = std::make_shared<CUDAMPLib::SingleArmSpace>(
+            dof, constraints, jt, poses, axes, par, smap, spos, rad, act, lo, hi, def, names, 0.02);
+        std::vector<std::vector<float>> start_joint_values_set{std::vector<float>(m[k + "start"].begin(), m[k + "start"].end())};
+        std::vector<std::vector<float>> goal_joint_values_set{std::vector<float>(m[k + "goal"].begin(), m[k + "goal"].end())};
+        CUDAMPLib::SingleArmTaskPtr problem_task =
+            std::make_shared<CUDAMPLib::SingleArmTask>(start_joint_values_set, goal_joint_values_set);
+        CUDAMPLib::RRGPtr planner = std::make_shared<CUDAMPLib::RRG>(single_arm_space);
+        planner->setMaxTravelDistance(6.0);
+        planner->setSampleAttemptsInEachIteration(60);
+        planner->setMotionTask(problem_task, false);
+        auto start_time = std::chrono::high_resolution_clock::now();
+        CUDAMPLib::TimeoutTerminationPtr termination_condition = std::make_shared<CUDAMPLib::TimeoutTermination>(10.0);
+        planner->solve(termination_condition);
+        auto end_time = std::chrono::high_resolution_clock::now();
+        auto time_taken = std::chrono::duration_cast<std::chrono::milliseconds>(end_time - start_time).count();
+        std::printf("%ssolved %d %ld\n", k.c_str(), (int)problem_task->hasSolution(), (long)time_taken);
+        print_path((k + "path").c_str(), problem_task->getSolution());
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc < 2) return 2;
     auto m = load(argv[1]);
+    const bool eval_mode = argc > 2 && std::string(argv[2]) == "evaluate";
     const int L = (int)m["joint_types"].size(), S = (int)m["sphere_radius"].size();
     std::vector<int> jt, par, smap;
     std::vector<mpx::Isometry3d> poses(L);
@@ -74,6 +136,7 @@ int main(int argc, char** argv) {
     }
     size_t dof = 0;
     for (bool b : act) dof += b;
+    if (eval_mode) return evaluate(m, jt, poses, axes, par, smap, spos, rad, act, lo, hi, def, names, en, dof);
     std::vector<float> start(m["start"].begin(), m["start"].end()), goal(m["goal"].begin(), m["goal"].end());
     const V& bx = m["box"];  // x_min x_max y_min y_max z_min z_max x y z roll pitch yaw
 
@@ -233,5 +296,17 @@ int main(int argc, char** argv) {
     planner0->setMotionTask(task0, true);
     planner0->solve(std::make_shared<CUDAMPLib::StepTermination>(0));
     std::printf("step0 %d %s\n", (int)task0->hasSolution(), task0->getFailureReason().c_str());
+
+    // cRRTC to a solution (cuda_test.cpp TEST_Planner with the cRRTC planner): the full path is
+    // densified to the space's resolution; without it, the planner's waypoints
+    for (int full = 1; full >= 0; full--) {
+        auto task_c = std::make_shared<CUDAMPLib::SingleArmTask>(std::vector<std::vector<float>>{start},
+                                                                 std::vector<std::vector<float>>{goal});
+        auto planner_c = std::make_shared<CUDAMPLib::cRRTC>(space);
+        planner_c->setMotionTask(task_c, full == 1);
+        planner_c->solve(std::make_shared<CUDAMPLib::TimeoutTermination>(10.0f));
+        std::printf("crrtc_solved_%d %d\n", full, (int)task_c->hasSolution());
+        print_path(full ? "crrtc_full_path" : "crrtc_path", task_c->getSolution());
+    }
     return 0;
 }

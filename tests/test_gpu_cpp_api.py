@@ -105,3 +105,70 @@ def test_cpp_api(tmp_path):
     assert res["rrg_invalid"] == ["0", "InvalidInput"]
     assert res["rrg_step0"] == ["0", "MeetTerminationCondition"]
     assert res["step0"] == ["0", "MeetTerminationCondition"]
+    # cRRTC to a solution, planned twice: the waypoints keep the planner's contract; the second
+    # solve returns its path densified at the space's resolution (interpolateVectors: floor(d / res)
+    # equal steps per segment, so every step is below 2 res).  The two solves run in the shared
+    # mode, which is not deterministic, so their waypoints may differ.  Both are valid at 0.005 rad.
+    assert res["crrtc_solved_1"] == ["1"] and res["crrtc_solved_0"] == ["1"]
+    pc = path("crrtc_path")
+    assert len(pc) >= 2 and np.array_equal(pc[0], start) and np.array_equal(pc[-1], goal)
+    assert oracle.validate_path(ro, so, pc, 0.005)[0]
+    pf = path("crrtc_full_path")
+    assert np.array_equal(pf[0], start) and np.array_equal(pf[-1], goal)
+    steps = np.linalg.norm(np.diff(pf.astype(np.float64), axis=0), axis=1)
+    assert np.all(steps < 2 * 0.02) and np.sum(steps) >= 1.0 - 1e-5  # start and goal are 1 apart
+    assert oracle.validate_path(ro, so, pf, 0.005)[0]
+
+
+def test_cpp_evaluate_cudampl_rrg_fr3(fr3, tmp_path):
+    """The RRG evaluation of src/foliation_planner/src/evaluate_cudampl.cpp:160-250 compiled against
+    this library with the reference's own include block: FR3, self collision plus 20 sphere
+    obstacles per task, travel distance 6.0, 60 samples per iteration, 10 s timeout.  Every path
+    starts and ends at the task's start and goal, and each edge passes the oracle's motion check
+    at the space's resolution (in one of its two directions) and the 0.005 rad re-check."""
+    from isaac_sim_ros2_humble_amd import native, workloads
+    native.init(0)
+    robot = native.Robot(fr3)
+    rng = np.random.default_rng(11)
+    tasks, extra = [], {}
+    while len(tasks) < 4:
+        c = rng.uniform([-0.8, -0.8, 0.0], [0.8, 0.8, 1.2], size=(20, 3))
+        keep = np.hypot(c[:, 0], c[:, 1]) > 0.3  # clear of the base column
+        c, r = c[keep], rng.uniform(0.03, 0.1, size=keep.sum())
+        scene = Scene(spheres=np.concatenate([c, r[:, None]], axis=1).astype(np.float32))
+        ns = native.Scene(scene)
+        pair = workloads.sample_pair(fr3, scene, rng, lambda m, s, q: native.check_states(robot, ns, q),
+                                     lambda m, s, a, b: native.check_edges(robot, ns, a, b, 0.02)[0],
+                                     want_free_line=False)
+        if pair is None:
+            continue
+        k = "task%d_" % len(tasks)
+        extra[k + "obstacles"] = c.reshape(-1)
+        extra[k + "radius"] = r
+        extra[k + "start"], extra[k + "goal"] = pair
+        tasks.append((scene, pair))
+    f = tmp_path / "fr3.txt"
+    dump(fr3, str(f), extra)
+    out = subprocess.run([BIN, str(f), "evaluate"], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr[-2000:]
+    res = {}
+    for line in out.stdout.splitlines():
+        k, *v = line.split()
+        res[k] = v
+    ro = oracle.OracleRobot(fr3)
+    solved = 0
+    for i, (scene, (s, g)) in enumerate(tasks):
+        ok, ms = (int(x) for x in res["task%d_solved" % i])
+        print("task %d: solved %d in %d ms" % (i, ok, ms))
+        if not ok:
+            continue
+        solved += 1
+        n = int(res["task%d_path" % i][0])
+        w = np.array([float(x) for x in res["task%d_path" % i][1:]], np.float32).reshape(n, fr3.dof)
+        assert np.array_equal(w[0], s) and np.array_equal(w[-1], g)
+        so = oracle.OracleScene(scene)
+        assert oracle.check(ro, so, w)[0].all()
+        fwd = oracle.edges(ro, so, w[:-1], w[1:], 0.02)[0]
+        bwd = oracle.edges(ro, so, w[1:], w[:-1], 0.02)[0]
+        assert np.all((fwd != 0) | (bwd != 0))
+    assert solved >= 3

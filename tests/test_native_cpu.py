@@ -99,7 +99,7 @@ def test_ctypes_structs_match_the_header(tmp_path):
     with the host C compiler from include/mp_abi.h)."""
     pairs = [(native.RobotDesc, "mp_robot_desc"), (native.RRTCParams, "mp_rrtc_params"), (native.Path, "mp_path"),
              (native.TuneResult, "mp_tune_result"), (native.ServerParams, "mp_server_params"),
-             (native.ServerStats, "mp_server_stats")]
+             (native.ServerStats, "mp_server_stats"), (native.RoadmapInfo, "mp_roadmap_info")]
     body = []
     for py, c in pairs:
         body.append('printf("%%zu", sizeof(%s));' % c)

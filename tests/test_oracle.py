@@ -539,3 +539,79 @@ def test_rrtconnect_path_invariants(rng):
         assert np.all(seg <= 0.3 * (1 + 1e-5)), seg.max()
         assert oracle.validate_path(rs, so, path, 0.02)[0]
     assert solved >= 10
+
+
+# ------------------------------------------------------------------------------------- roadmap
+def test_rrg_free_space_closed_form():
+    """Free space: every sample of the first pass reaches both groups, so the first pass solves.
+    Each added vertex is sample i pulled to within the travel distance of the start, with an edge
+    to the start and one to the goal weighted by the motion lengths."""
+    m = point_robot(0.02, lim=1.0)
+    ro, so = oracle.OracleRobot(m), oracle.OracleScene(Scene())
+    s, g = np.array([[-0.5, 0.0, 0.0]], np.float32), np.array([[0.5, 0.2, 0.0]], np.float32)
+    k, travel = 16, 0.3
+    out = oracle.rrg(ro, so, s, g, k, travel, 0.02, seed=7, offset=100, max_passes=50)
+    assert out["solved"] == 1 and out["passes"] == 1 and out["loop_index"] == 1
+    assert out["nv"] == 2 + k and out["ne"] == 2 * k
+    u = ro.sample(7, 100, k).astype(np.float64)
+    d = np.linalg.norm(u - s[0], axis=1)
+    want = s[0] + (u - s[0]) * np.minimum(1.0, travel / d)[:, None]
+    v = out["vertices"][2:].astype(np.float64)
+    assert np.allclose(v, want, atol=1e-6)
+    assert list(out["groups"]) == [1, 2] + [0] * k
+    e = out["edges"]
+    assert list(e[:, 0]) == [2 + i // 2 for i in range(2 * k)] and list(e[:, 1]) == [0, 1] * k
+    assert np.allclose(out["weights"][0::2], np.linalg.norm(v - s[0], axis=1), rtol=1e-6)
+    assert np.allclose(out["weights"][1::2], np.linalg.norm(v - g[0], axis=1), rtol=1e-6)
+
+
+def test_rrg_halves_travel_and_keeps_the_side():
+    """The start enclosed by a closed box shell: no sample outside reaches it, so each pass halves
+    the start group's travel distance (floor 0.1) and the loop stays on the start side (the loop
+    index advances by two) until the samples are pulled inside the shell."""
+    m = point_robot(0.02, lim=1.0)
+    ro = oracle.OracleRobot(m)
+    walls = [box_record([dx, dy, dz], np.eye(3), h) for dx, dy, dz, h in [
+        (-0.12, 0, 0, [0.01, 0.13, 0.13]), (0.12, 0, 0, [0.01, 0.13, 0.13]), (0, -0.12, 0, [0.13, 0.01, 0.13]),
+        (0, 0.12, 0, [0.13, 0.01, 0.13]), (0, 0, -0.12, [0.13, 0.13, 0.01]), (0, 0, 0.12, [0.13, 0.13, 0.01])]]
+    so = oracle.OracleScene(Scene(boxes=np.array(walls)))
+    s, g = np.zeros((1, 3), np.float32), np.array([[0.8, 0.8, 0.8]], np.float32)
+    for passes, travel in ((1, 3.0), (2, 1.5), (3, 0.75), (4, 0.375), (5, 0.1875), (6, 0.1)):
+        out = oracle.rrg(ro, so, s, g, 8, 6.0, 0.02, seed=3, offset=0, max_passes=passes)
+        assert out["travel_start"] == np.float32(travel) and out["loop_index"] == 2 * passes, (passes, out["travel_start"])
+        assert out["nv"] == 2 and out["ne"] == 0 and out["travel_goal"] == np.float32(6.0)
+    # within 0.1 of the start the samples land inside the shell and join the start group
+    out = oracle.rrg(ro, so, s, g, 8, 6.0, 0.02, seed=3, offset=0, max_passes=12)
+    assert out["nv"] > 2 and out["start_group"] > 1
+    assert np.all(np.abs(out["vertices"][out["groups"] == 1]) < 0.11)
+
+
+def test_rrg_graph_invariants(rng):
+    """In clutter: every vertex is free; every edge runs from an added vertex to an earlier vertex
+    of a group, its motion is free at the resolution and its weight is the motion length; the
+    member lists hold exactly the vertices of each group in order; a vertex of group 0 exists only
+    when solved and reaches both groups."""
+    m = point_robot(0.03, lim=1.0)
+    ro = oracle.OracleRobot(m)
+    boxes = [box_record(rng.uniform(-0.7, 0.7, 3), random_rotation(rng), rng.uniform(0.05, 0.2, 3)) for _ in range(14)]
+    scene = Scene(boxes=np.array(boxes))
+    so = oracle.OracleScene(scene)
+    q = rng.uniform(-1, 1, size=(256, 3)).astype(np.float32)
+    free = q[oracle.check(ro, so, q)[0] == 1]
+    out = oracle.rrg(ro, so, free[:2], free[2:3], 24, 1.0, 0.02, seed=5, offset=0, max_passes=60)
+    v, grp, e, w = out["vertices"], out["groups"], out["edges"], out["weights"]
+    assert oracle.check(ro, so, v)[0].all()
+    assert np.all(e[:, 0] > e[:, 1]) and np.all(e[:, 0] >= 3)
+    ok, ln = oracle.edges(ro, so, v[e[:, 0]], v[e[:, 1]], 0.02)
+    assert ok.all() and np.array_equal(ln, w)
+    assert list(out["start_members"]) == list(np.nonzero(grp == 1)[0])
+    assert list(out["goal_members"]) == list(np.nonzero(grp == 2)[0])
+    assert np.all(grp[e[:, 1]] != 0)
+    if out["solved"]:
+        j = np.nonzero(grp == 0)[0]
+        assert len(j) >= 1
+        for x in j:
+            assert set(grp[e[e[:, 0] == x, 1]]) == {1, 2}
+    else:
+        assert not (grp == 0).any()
+    assert out["nv"] > 10
