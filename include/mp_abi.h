@@ -301,6 +301,20 @@ typedef struct {
 int mp_rrtc_solve(mp_robot* robot, const mp_scene* scene, const float* start, const float* goal,
                   const mp_rrtc_params* params, mp_path* out);
 
+/* mp_rrtc_solve without allocating: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops, harnesses that forbid
+ * allocation).  The query's trees, parents and domain radii, descriptor, state, control block and
+ * path live in a static device array of MP_NOALLOC_SCRATCH_BYTES per device (held until the
+ * process ends; its address is looked up at mp_init); the call allocates nothing and copies the
+ * path into out->waypoints.  Same semantics and results as mp_rrtc_solve (exclusive mode: the
+ * same path bit for bit).  MP_E_CAPACITY when 2 x tree_capacity x dof floats of trees, 4 x
+ * tree_capacity ints of parents and radii, and max_path x dof floats of path (each rounded up to
+ * 256 B, plus 1 KiB of descriptors) exceed the array.  Calls on one device share the array: the
+ * library serialises them, and each clears what must start empty. */
+#define MP_NOALLOC_SCRATCH_BYTES ((size_t)4 << 20)
+int mp_rrtc_solve_noalloc(mp_robot* robot, const mp_scene* scene, const float* start, const float* goal,
+                          const mp_rrtc_params* params, mp_path* out);
+
 /* nq independent queries (robots may differ) in ONE persistent launch with per-query work
  * queues; robots[k], scenes[k], starts[k] / goals[k] ([dof_k] floats at offset k*16). */
 int mp_rrtc_solve_batch(int nq, mp_robot* const* robots, const mp_scene* const* scenes, const float* starts16,

@@ -30,12 +30,34 @@ def embed_template():
             f.write(new)
 
 
+HIP_SOURCES = ["noalloc.hip"]  # sources with device code of their own, compiled by hipcc
+OBJ_DIR = os.path.join(HERE, "_build")
+
+
+def compile_hip_objects(verbose=False):
+    """hipcc -c for gfx950 of the library's own device-code sources (the no-allocation entry
+    point's scratch array); returns the object paths."""
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
+        cmd = [os.path.join(ROCM, "bin", "hipcc"), "-c", "-fPIC", "-O2", "-std=c++17", "--offload-arch=gfx950",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=off",
+               "-I" + os.path.join(REPO, "include"), "-I" + CSRC, os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd))
+        subprocess.check_call(cmd)
+        objs.append(obj)
+    return objs
+
+
 def compile_lib(verbose=False):
     embed_template()
+    objs = compile_hip_objects(verbose)
     cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-Wall",
            "-Wno-unused-function", "-Wno-unused-result", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(REPO, "include"), "-I" + CSRC,
            "-I" + os.path.join(ROCM, "include")]
-    cmd += [os.path.join(CSRC, s) for s in SOURCES]
+    cmd += [os.path.join(CSRC, s) for s in SOURCES] + objs
     cmd += ["-o", LIB + ".tmp", "-L" + os.path.join(ROCM, "lib"), "-Wl,-rpath," + os.path.join(ROCM, "lib"),
             "-lamdhip64", "-lhiprtc", "-ldl", "-lpthread"]
     if verbose:

@@ -165,6 +165,7 @@ int mp_init(int device) {
         MP_HIP(hipGetDeviceCount(&n));
         if (device < 0 || device >= n) throw Error(MP_E_INVALID, "no such HIP device");
         MP_HIP(hipSetDevice(device));
+        mp::noalloc_setup(device);  // the no-allocation entry point's scratch address on this device
         MP_HIP(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
         hipDeviceProp_t p;
         MP_HIP(hipGetDeviceProperties(&p, device));

@@ -137,6 +137,10 @@ struct LaunchPlan {
 LaunchPlan plan_launch(int nq, mp_robot* const* robots, const mp_scene* const* scenes, const mp_rrtc_params* params,
                        int default_wave_count);
 
+// The per-device address of the no-allocation entry point's scratch array, looked up once per
+// device with that device current (called by mp_init).
+void noalloc_setup(int device);
+
 // Launch shape of the planning kernel: waves per team, total waves (0: resident teams) and
 // teams per query (0: unbounded).
 struct Shape {

@@ -114,6 +114,7 @@ _SIGS = {
     "mp_rrtc_default_params": (None, [C.POINTER(RRTCParams)]),
     "mp_rrtc_solve": (C.c_int, [P, P, P, P, C.POINTER(RRTCParams), C.POINTER(Path)]),
     "mp_rrtc_solve_batch": (C.c_int, [C.c_int, P, P, P, P, C.POINTER(RRTCParams), P]),
+    "mp_rrtc_solve_noalloc": (C.c_int, [P, P, P, P, C.POINTER(RRTCParams), C.POINTER(Path)]),
     "mp_rrtc_autotune": (C.c_int, [C.c_int, P, P, P, P, C.POINTER(RRTCParams), C.c_int, C.c_int, P, C.c_int,
                                    C.POINTER(C.c_int), C.POINTER(RRTCParams)]),
     "mp_roadmap_create": (C.c_int, [P, P, P, C.c_int, P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, C.c_uint64,
