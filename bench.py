@@ -254,7 +254,8 @@ def serve(robots, lib, rate, seconds, seed, sample_paths=1000, saturation_querie
            "submitted": st["submitted"], "completed": st["completed"], "solved": st["solved"],
            "success_rate": st["solved"] / max(st["completed"], 1), "scene_uploads": uploads,
            "throughput_qps": st["throughput_qps"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
-           "mean_ms": st["mean_ms"], "steps": st["steps"], "statuses": statuses, "submit_seconds": submit_s}
+           "mean_ms": st["mean_ms"], "steps": st["steps"], "worker_step_ms": st["step_ms"],
+           "worker_loop_ms": st["loop_ms"], "statuses": statuses, "submit_seconds": submit_s}
     return out, paths
 
 

@@ -403,6 +403,8 @@ typedef struct {
     int active, queued;
     double p50_ms, p99_ms, mean_ms, throughput_qps;
     int64_t steps;
+    double step_ms;  /* worker time inside steps: enqueue, device work, synchronisation */
+    double loop_ms;  /* worker time in whole steps: admission, the step, retiring finished queries */
 } mp_server_stats;
 
 /* A server thread owns the device; submit/poll are thread-safe. */

@@ -269,6 +269,7 @@ std::vector<SphereLink> sphere_links(const RobotModel& m) {
 
 // flop costs of the planning cost model (FMA = 2, add/mul/min/max/cmp = 1)
 constexpr int kXform = 18, kSelf = 9, kSph = 11, kBox = 34, kCyl = 35, kCap = 28;
+constexpr int kCentreChunk = 16;  // spheres per staged chunk; the LDS row is MP_CENTRE_ROW floats
 
 }  // namespace
 
@@ -395,6 +396,35 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
              << "; out[" << 3 * s + 2 << "] = " << named(C, c[2]) << ";\n";
     }
     std::string centres_body = C.os.str();
+
+    // ---------------------------------------------------------------- staged centres output
+    // The wave's centres [64][S][3] are one contiguous run of memory.  Written lane by lane, every
+    // store instruction touches 64 lines 12 S bytes apart, and lines leave the cache before they
+    // are complete.  Staged: chunks of kCentreChunk spheres go to the wave's LDS tile (one row of
+    // 3 kCentreChunk + 1 floats per lane), and the wave copies each chunk out with consecutive
+    // lanes on consecutive floats.  The values are the same expressions as write_centres.
+    std::ostringstream staged;
+    for (int s0 = 0; s0 < m.S; s0 += kCentreChunk) {
+        const int s1 = std::min(m.S, s0 + kCentreChunk), cc = 3 * (s1 - s0);
+        Emitter X;
+        X.pfx = "c" + std::to_string(s0) + "_";
+        X.ind = "            ";
+        X.os << "        {\n";
+        for (int s = s0; s < s1; s++) {
+            V c[3];
+            centre(X, s, c);
+            for (int a = 0; a < 3; a++) X.os << X.ind << "row[" << 3 * (s - s0) + a << "] = " << named(X, c[a]) << ";\n";
+        }
+        X.os << "        }\n";
+        X.os << "        mp_wave_sync();\n";
+        X.os << "        for (int t = lane; t < live * " << cc << "; t += MP_WAVE) {\n"
+             << "            const int L = t / " << cc << ", j = t - L * " << cc << ";\n"
+             << "            out[(long long)L * " << 3 * m.S << " + " << 3 * s0 << " + j] = tile[L * MP_CENTRE_ROW + j];\n"
+             << "        }\n"
+             << "        mp_wave_sync();\n";
+        staged << X.os.str();
+    }
+    std::string staged_body = staged.str();
 
     // ---------------------------------------------------------------- self collision blocks
     // link pairs in the order of the first sphere index of each link (the oracle's order)
@@ -660,6 +690,12 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
     o << "    static constexpr int NLINK = " << m.L << ";\n";
     o << "    static __device__ __forceinline__ void poses(const float* q, float* P) {\n" << poses_body << "    }\n";
     o << "    static __device__ __forceinline__ void write_centres(const float* F, float* out) {\n" << centres_body << "    }\n";
+    o << "    // out: the wave's first configuration; live: its configurations below n; tile: the wave's\n"
+         "    // LDS tile of MP_WAVE x MP_CENTRE_ROW floats\n"
+         "    static __device__ __forceinline__ void write_centres_staged(const float* F, float* out, int live, float* tile) {\n"
+         "        const int lane = threadIdx.x & (MP_WAVE - 1);\n"
+         "        float* row = tile + lane * MP_CENTRE_ROW;\n"
+      << staged_body << "    }\n";
     o << "    template <int MODE>\n    static __device__ __forceinline__ bool self_hit(const float* F, bool h, u64 mask, unsigned& work, const Team& T) {\n"
       << self_body(false) << "        return h;\n    }\n";
     o << "    static __device__ __forceinline__ float self_cost(const float* F, unsigned& work) {\n        float part = 0.0f;\n"
@@ -896,12 +932,14 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "                qi = mp_uniform(qi);\n"
          "                // every running query it saw has its teams: this team is surplus and leaves\n"
          "                if (qi < 0 && full) qi = -2;\n"
+
          "            }\n"
          "        }\n"
          "        // wave-uniform: the descriptor and scene loads below then use scalar loads\n"
          "        qi = mp_uniform(mp_team_bcast(qi, T));\n"
          "        if (qi == -2) break;\n"
          "        if (qi < 0) { mp_prof_since(pl, MP_PROF_SCHED, ts); continue; }\n"
+
          "        QueryState* st = ctl.st + qi;\n"
          "        QueryDev Q;\n"
          "        mp_load_query(ctl.q + qi, Q);\n"

@@ -9,6 +9,7 @@
 // step (control-block copy, admission kernel, planning kernel, state copy) can be captured once
 // in a hipGraph and replayed.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -83,16 +84,25 @@ struct mp_server {
     std::vector<double> latencies;
     double t_first = -1.0, t_last = 0.0;
 
+    // worker-thread time in step() (enqueue and device) and in whole steps (admission, step,
+    // retiring); written by the worker only
+    std::atomic<double> busy_ms{0.0}, loop_ms{0.0};
+
     void run();
-    void step(const std::vector<int>& list, int n_new);
+    void step(const std::vector<int>& list, int n_new, bool backlog);
 };
 
-void mp_server::step(const std::vector<int>& list, int n_new) {
+// Queries waiting for a slot (backlog): the step runs kBacklogBudget times the iterations, so
+// the step's fixed costs (launch, copies, synchronisation, retiring) are spread over more work;
+// without a backlog short steps keep the latency low.
+constexpr int kBacklogBudget = 4;
+
+void mp_server::step(const std::vector<int>& list, int n_new, bool backlog) {
     int* h = pool->hpinned;
     h[0] = 0;
     h[1] = (int)list.size();
     h[2] = (int)list.size();
-    h[3] = budget;
+    h[3] = backlog ? kBacklogBudget * budget : budget;
     h[4] = (int)list.size() - n_new;
     h[5] = n_new;
     h[6] = prm.params.exclusive ? 1 : 0;
@@ -100,9 +110,16 @@ void mp_server::step(const std::vector<int>& list, int n_new) {
     h[8] = cap;
     h[9] = mp::pair_flag(lanes);
     for (size_t k = 0; k < list.size(); k++) h[16 + k] = list[k];
-    for (int k = (int)list.size() - n_new; k < (int)list.size(); k++) {
-        int slot = list[k];
-        MP_HIP(hipMemcpyAsync(pool->dq + slot, &pool->hq[slot], sizeof(mp::QueryDevHost), hipMemcpyHostToDevice, stream));
+    if (n_new > 0) {
+        // the new queries' descriptors in one copy of the slot range that holds them (the
+        // descriptors of running queries in that range are rewritten with the same bytes)
+        int lo = nslots, hi = -1;
+        for (int k = (int)list.size() - n_new; k < (int)list.size(); k++) {
+            lo = std::min(lo, list[k]);
+            hi = std::max(hi, list[k]);
+        }
+        MP_HIP(hipMemcpyAsync(pool->dq + lo, &pool->hq[lo], (size_t)(hi - lo + 1) * sizeof(mp::QueryDevHost),
+                              hipMemcpyHostToDevice, stream));
     }
     if (prm.use_graph) {
         if (!graph) {
@@ -139,11 +156,15 @@ void mp_server::run() {
     try {
         std::vector<int> list;
         std::vector<int> fresh;
+        std::vector<char> is_fresh;
+        double t_loop = 0.0;
         while (true) {
+            bool backlog = false;
             {
                 std::unique_lock<std::mutex> g(mu);
                 cv.wait(g, [&] { return stop || !queue.empty() || !active.empty(); });
                 if (stop) break;
+                t_loop = now_ms();
                 fresh.clear();
                 while (!queue.empty() && !free_slots.empty()) {
                     Pending p = queue.front();
@@ -156,13 +177,19 @@ void mp_server::run() {
                     active[slot] = ActiveQ{p.ticket, p.t_submit, r->model.dof, p.scene};
                     fresh.push_back(slot);
                 }
+                backlog = !queue.empty();
                 list.clear();
+                is_fresh.assign((size_t)nslots, 0);
+                for (int s : fresh) is_fresh[s] = 1;
                 for (auto& kv : active)
-                    if (std::find(fresh.begin(), fresh.end(), kv.first) == fresh.end()) list.push_back(kv.first);
+                    if (!is_fresh[kv.first]) list.push_back(kv.first);
                 for (int s : fresh) list.push_back(s);
             }
             if (list.empty()) continue;
-            step(list, (int)fresh.size());
+            const double t0 = now_ms();
+            step(list, (int)fresh.size(), backlog);
+            const double t1 = now_ms();
+            busy_ms.store(busy_ms.load(std::memory_order_relaxed) + (t1 - t0), std::memory_order_relaxed);
             steps++;
             // retire finished queries
             std::vector<std::pair<int64_t, Done>> fin;
@@ -177,9 +204,10 @@ void mp_server::run() {
                 }
                 Done d;
                 std::memset(&d.info, 0, sizeof(d.info));
-                d.waypoints.resize((size_t)pool->max_path * aq.dof);
+                const int need = st.status == MP_PLAN_SOLVED && st.path_len > 0 ? st.path_len : 0;
+                d.waypoints.resize((size_t)need * aq.dof);
                 d.info.waypoints = d.waypoints.data();
-                d.info.capacity = pool->max_path;
+                d.info.capacity = need;
                 mp::harvest(*pool, slot, aq.dof, st, &d.info);
                 fin.emplace_back(aq.ticket, std::move(d));
                 freed.push_back(slot);
@@ -204,6 +232,7 @@ void mp_server::run() {
                 }
             }
             idle_cv.notify_all();
+            loop_ms.store(loop_ms.load(std::memory_order_relaxed) + (now_ms() - t_loop), std::memory_order_relaxed);
         }
     } catch (const std::exception& e) {
         std::lock_guard<std::mutex> g(mu);
@@ -359,6 +388,8 @@ int mp_server_stats_get(mp_server* s, mp_server_stats* out) {
         out->active = (int)s->active.size();
         out->queued = (int)s->queue.size();
         out->steps = s->steps;
+        out->step_ms = s->busy_ms.load(std::memory_order_relaxed);
+        out->loop_ms = s->loop_ms.load(std::memory_order_relaxed);
         if (!s->latencies.empty()) {
             std::vector<double> v = s->latencies;
             std::sort(v.begin(), v.end());
