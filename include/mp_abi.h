@@ -130,6 +130,21 @@ int mp_scene_create(const float* spheres, int ns, const float* boxes, int nb, co
  * printed and returned). */
 int mp_scene_update(mp_scene* scene, const float* spheres, int ns, const float* boxes, int nb,
                     const float* cylinders, int nc, const float* capsules, int ncap);
+/* Joint bounds of every check, cost and plan on this scene.  In CUDAMPLib, joint bounds are a
+ * constraint of the space like the obstacles: SingleArmSpace::checkStates evaluates a
+ * BoundaryConstraint only when the space holds one (SingleArmSpace.cu:1572-1602; its cost,
+ * BoundaryConstraint.cu:56-92: the squared violations of the active joints summed, valid below
+ * 1e-4).  A scene is that constraint set, so it carries the bounds:
+ *   MP_BOUNDS_ROBOT   the robot's joint limits (the default);
+ *   MP_BOUNDS_NONE    no joint-bound check (a space without a BoundaryConstraint);
+ *   MP_BOUNDS_CUSTOM  lower[n], upper[n] over the active joints (a BoundaryConstraint's own
+ *                     bounds); joints from n on are unbounded, and +-INFINITY leaves a side open.
+ * The cost row MP_COST_BOUNDS follows the same bounds.  MP_E_STATE while server queries use the
+ * scene. */
+#define MP_BOUNDS_ROBOT 0
+#define MP_BOUNDS_NONE 1
+#define MP_BOUNDS_CUSTOM 2
+int mp_scene_set_joint_bounds(mp_scene* scene, int mode, const float* lower, const float* upper, int n);
 int mp_scene_destroy(mp_scene* scene);
 
 /* ---------------------------------------------------------------------------------------- */
@@ -309,7 +324,7 @@ int mp_rrtc_solve(mp_robot* robot, const mp_scene* scene, const float* start, co
  * path into out->waypoints.  Same semantics and results as mp_rrtc_solve (exclusive mode: the
  * same path bit for bit).  MP_E_CAPACITY when 2 x tree_capacity x dof floats of trees, 4 x
  * tree_capacity ints of parents and radii, and max_path x dof floats of path (each rounded up to
- * 256 B, plus 1 KiB of descriptors) exceed the array.  Calls on one device share the array: the
+ * 256 B, plus 1.25 KiB of descriptors) exceed the array.  Calls on one device share the array: the
  * library serialises them, and each clears what must start empty. */
 #define MP_NOALLOC_SCRATCH_BYTES ((size_t)4 << 20)
 int mp_rrtc_solve_noalloc(mp_robot* robot, const mp_scene* scene, const float* start, const float* goal,

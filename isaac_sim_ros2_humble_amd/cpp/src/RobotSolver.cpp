@@ -47,6 +47,9 @@ void RobotSolver::setEnvObstacleCache(int num_of_spheres, int num_of_cuboids, in
     mpcpp::check(mp_scene_create(s.data(), cap_s_, b.data(), cap_c_, c.data(), cap_y_, nullptr, 0, &scene_),
                  "mp_scene_create");
     mpcpp::check(mp_scene_update(scene_, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, 0), "mp_scene_update");
+    // the CPRRTC kernel checks no joint limits (RobotSolver.cu:507-799): its samples are drawn
+    // inside them (RobotSolver.cu:20-36) and its motions stay between checked states
+    mpcpp::check(mp_scene_set_joint_bounds(scene_, MP_BOUNDS_NONE, nullptr, nullptr, 0), "mp_scene_set_joint_bounds");
 }
 
 void RobotSolver::updateEnvObstacle(std::vector<Sphere>& spheres, std::vector<Cuboid>& cuboids,

@@ -215,6 +215,24 @@ SingleArmSpace::SingleArmSpace(size_t dim, const std::vector<BaseConstraintPtr>&
     mpcpp::check(mp_scene_create(s.data(), (int)(s.size() / 4), b.data(), (int)(b.size() / 18), c.data(),
                                  (int)(c.size() / 14), nullptr, 0, &scene_),
                  "mp_scene_create");
+    // joint bounds are checked only through a BoundaryConstraint of the space, with that
+    // constraint's own bounds (SingleArmSpace.cu:1572-1602, BoundaryConstraint.cu:56-92)
+    BoundaryConstraintPtr bc;
+    for (const auto& con : constraints)
+        if (auto p = std::dynamic_pointer_cast<BoundaryConstraint>(con)) bc = p;
+    if (!bc) {
+        mpcpp::check(mp_scene_set_joint_bounds(scene_, MP_BOUNDS_NONE, nullptr, nullptr, 0), "mp_scene_set_joint_bounds");
+    } else {
+        std::vector<float> lo, hi;
+        for (size_t j = 0; j < active_joint_map.size(); j++) {
+            if (!active_joint_map[j]) continue;
+            const bool on = j < bc->active_.size() && bc->active_[j] && j < bc->lower_.size() && j < bc->upper_.size();
+            lo.push_back(on ? bc->lower_[j] : -INFINITY);
+            hi.push_back(on ? bc->upper_[j] : INFINITY);
+        }
+        mpcpp::check(mp_scene_set_joint_bounds(scene_, MP_BOUNDS_CUSTOM, lo.data(), hi.data(), (int)lo.size()),
+                     "mp_scene_set_joint_bounds");
+    }
 }
 
 SingleArmSpace::~SingleArmSpace() {

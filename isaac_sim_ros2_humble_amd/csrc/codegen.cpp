@@ -839,7 +839,7 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "    Q.scene.ns = mp_sld(&p->scene.ns); Q.scene.nb = mp_sld(&p->scene.nb); Q.scene.nc = mp_sld(&p->scene.nc);\n"
          "    Q.scene.ncap = mp_sld(&p->scene.ncap);\n"
          "    Q.scene.sph = mp_sld(&p->scene.sph); Q.scene.box = mp_sld(&p->scene.box); Q.scene.cyl = mp_sld(&p->scene.cyl);\n"
-         "    Q.scene.cap = mp_sld(&p->scene.cap);\n"
+         "    Q.scene.cap = mp_sld(&p->scene.cap); Q.scene.bnd = mp_sld(&p->scene.bnd);\n"
          "    Q.tree[0] = mp_sld(&p->tree[0]); Q.tree[1] = mp_sld(&p->tree[1]);\n"
          "    Q.parent[0] = mp_sld(&p->parent[0]); Q.parent[1] = mp_sld(&p->parent[1]); Q.path = mp_sld(&p->path);\n"
          "    Q.seed = mp_sld(&p->seed); Q.robot = mp_sld(&p->robot); Q.dof = mp_sld(&p->dof); Q.cap = mp_sld(&p->cap);\n"

@@ -87,6 +87,7 @@ SceneDevHost scene_view(const mp_scene* sc) {
     v.box = sc->dev + sc->offs[1];
     v.cyl = sc->dev + sc->offs[2];
     v.cap = sc->dev + sc->offs[3];
+    v.bnd = sc->bounds_mode == MP_BOUNDS_ROBOT ? nullptr : sc->bnd_dev;
     return v;
 }
 
@@ -535,15 +536,39 @@ int mp_scene_update(mp_scene* sc, const float* spheres, int ns, const float* box
     });
 }
 
+int mp_scene_set_joint_bounds(mp_scene* sc, int mode, const float* lower, const float* upper, int n) {
+    return guard([&] {
+        need_init();
+        if (!sc) throw Error(MP_E_INVALID, "null scene");
+        if (mode != MP_BOUNDS_ROBOT && mode != MP_BOUNDS_NONE && mode != MP_BOUNDS_CUSTOM)
+            throw Error(MP_E_INVALID, "joint-bound mode must be MP_BOUNDS_ROBOT, MP_BOUNDS_NONE or MP_BOUNDS_CUSTOM");
+        if (mode == MP_BOUNDS_CUSTOM && (n < 0 || n > 16 || (n > 0 && (!lower || !upper))))
+            throw Error(MP_E_INVALID, "custom joint bounds: n in [0, 16] with lower and upper arrays");
+        if (sc->server_refs.load() > 0)
+            throw Error(MP_E_STATE, "the scene has server queries queued or running; change it after they finish");
+        float h[32];
+        for (int j = 0; j < 16; j++) {
+            const bool given = mode == MP_BOUNDS_CUSTOM && j < n;
+            h[j] = given ? lower[j] : -INFINITY;
+            h[16 + j] = given ? upper[j] : INFINITY;
+        }
+        MP_HIP(hipStreamSynchronize(G.stream));
+        if (mode != MP_BOUNDS_ROBOT) {
+            if (!sc->bnd_dev) MP_HIP(hipMalloc((void**)&sc->bnd_dev, sizeof(h)));
+            MP_HIP(hipMemcpy(sc->bnd_dev, h, sizeof(h), hipMemcpyHostToDevice));
+        }
+        sc->bounds_mode = mode;
+    });
+}
+
 int mp_scene_destroy(mp_scene* sc) {
     return guard([&] {
         if (!sc) return;
         if (sc->server_refs.load() > 0)
             throw Error(MP_E_STATE, "the scene has server queries queued or running; destroy it after they finish");
-        if (sc->dev) {
-            hipStreamSynchronize(G.stream);
-            hipFree(sc->dev);
-        }
+        if (sc->dev || sc->bnd_dev) hipStreamSynchronize(G.stream);
+        if (sc->dev) hipFree(sc->dev);
+        if (sc->bnd_dev) hipFree(sc->bnd_dev);
         delete sc;
     });
 }

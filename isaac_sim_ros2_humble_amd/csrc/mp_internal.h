@@ -98,6 +98,10 @@ struct mp_scene {
     int cap_s = 0, cap_b = 0, cap_c = 0, cap_k = 0;
     float* dev = nullptr;  // one allocation: spheres | boxes | cylinders | capsules
     size_t offs[4] = {0, 0, 0, 0};
+    // joint bounds of every check on this scene: MP_BOUNDS_ROBOT (the robot's limits), or the
+    // 32 floats lower[16] | upper[16] in bnd_dev
+    int bounds_mode = 0;
+    float* bnd_dev = nullptr;
     // queries of servers that are queued or active on this scene: while > 0 the scene's device
     // memory is read by a server's stream, so update and destroy are refused
     mutable std::atomic<int> server_refs{0};

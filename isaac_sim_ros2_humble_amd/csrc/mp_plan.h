@@ -18,6 +18,7 @@ struct SceneDevHost {
     const float* box;
     const float* cyl;
     const float* cap;
+    const float* bnd;  // null: the robot's joint limits; else lower[16], upper[16]
 };
 
 struct QueryDevHost {
@@ -31,7 +32,7 @@ struct QueryDevHost {
     float start[16];
     float goal[16];
 };
-static_assert(sizeof(QueryDevHost) == 256, "QueryDev layout");
+static_assert(sizeof(QueryDevHost) == 264, "QueryDev layout");
 
 struct QueryStateHost {
     int alloc[2];

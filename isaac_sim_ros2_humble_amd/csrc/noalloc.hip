@@ -33,8 +33,10 @@ size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Offsets of one query's buffers in the array.
 struct Layout {
-    size_t ctl = 0, active = 256, q = 512, st = 768, trees = 1024, parents = 0, path = 0, end = 0;
+    size_t ctl = 0, active = 256, q = 512, st = 1024, trees = 1280, parents = 0, path = 0, end = 0;
 };
+static_assert(sizeof(mp::QueryDevHost) <= 1024 - 512 && sizeof(mp::QueryStateHost) <= 1280 - 1024,
+              "descriptor slots of the scratch layout");
 Layout layout(int cap, int dof, int max_path) {
     Layout L;
     L.parents = up256(L.trees + (size_t)2 * dof * cap * sizeof(float));

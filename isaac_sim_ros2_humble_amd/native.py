@@ -38,6 +38,7 @@ class RRTCParams(C.Structure):
                 ("balance", C.c_int), ("tree_ratio", C.c_float)]
 
 
+BOUNDS_ROBOT, BOUNDS_NONE, BOUNDS_CUSTOM = 0, 1, 2  # mp_scene_set_joint_bounds modes
 BALANCE_SMALLER, BALANCE_SPLIT, BALANCE_DYNAMIC, BALANCE_ALTERNATE = 0, 1, 2, 3
 
 
@@ -102,6 +103,7 @@ _SIGS = {
     "mp_scene_create": (C.c_int, [P, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int, C.POINTER(C.c_void_p)]),
     "mp_scene_update": (C.c_int, [P, P, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int]),
     "mp_scene_destroy": (C.c_int, [P]),
+    "mp_scene_set_joint_bounds": (C.c_int, [P, C.c_int, P, P, C.c_int]),
     "mp_fk_spheres": (C.c_int, [P, P, C.c_int64, P, P]),
     "mp_fk_link_poses": (C.c_int, [P, P, C.c_int64, P, P]),
     "mp_check_states": (C.c_int, [P, P, P, C.c_int64, P, P, P]),
@@ -266,13 +268,33 @@ class Scene:
         check(lib().mp_scene_create(_p(scene.spheres), ns, _p(scene.boxes), nb, _p(scene.cylinders), nc,
                                     _p(scene.capsules), ncap, C.byref(h)), "mp_scene_create")
         self.h = h.value
+        if getattr(scene, "joint_bounds", None) is not None:
+            self.set_joint_bounds(scene.joint_bounds)
 
     def update(self, scene):
         ns, nb, nc, ncap = scene.counts
         check(lib().mp_scene_update(self.h, _p(scene.spheres), ns, _p(scene.boxes), nb, _p(scene.cylinders), nc,
                                     _p(scene.capsules), ncap), "mp_scene_update")
+        self.set_joint_bounds(getattr(scene, "joint_bounds", None))
         self.scene = scene
         self._keep = [scene.spheres, scene.boxes, scene.cylinders, scene.capsules]
+
+    def set_joint_bounds(self, bounds):
+        """None: the robot's joint limits; "none": no joint-bound check; (lower, upper): bounds
+        over the active joints."""
+        if bounds is None:
+            check(lib().mp_scene_set_joint_bounds(self.h, BOUNDS_ROBOT, None, None, 0), "mp_scene_set_joint_bounds")
+        elif isinstance(bounds, str):
+            if bounds != "none":
+                raise ValueError("joint bounds: None, 'none' or (lower, upper)")
+            check(lib().mp_scene_set_joint_bounds(self.h, BOUNDS_NONE, None, None, 0), "mp_scene_set_joint_bounds")
+        else:
+            lo = np.ascontiguousarray(bounds[0], np.float32).reshape(-1)
+            hi = np.ascontiguousarray(bounds[1], np.float32).reshape(-1)
+            if lo.shape != hi.shape:
+                raise ValueError("joint bounds: lower and upper differ in length")
+            check(lib().mp_scene_set_joint_bounds(self.h, BOUNDS_CUSTOM, _p(lo), _p(hi), len(lo)),
+                  "mp_scene_set_joint_bounds")
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -22,6 +22,10 @@ class Scene:
     boxes: np.ndarray = field(default_factory=lambda: _empty(18))
     cylinders: np.ndarray = field(default_factory=lambda: _empty(14))
     capsules: np.ndarray = field(default_factory=lambda: _empty(7))
+    # joint bounds of the checks on this scene: None for the robot's limits, "none" for no
+    # joint-bound check (a space without a BoundaryConstraint), or (lower, upper) over the active
+    # joints (a BoundaryConstraint's own bounds; +-inf leaves a side open)
+    joint_bounds: object = None
 
     def __post_init__(self):
         self.spheres = np.ascontiguousarray(self.spheres, np.float32).reshape(-1, 4)

@@ -38,6 +38,7 @@ def lib():
         L.orc_scene_create.restype = P
         L.orc_scene_create.argtypes = [P, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int]
         L.orc_scene_destroy.argtypes = [P]
+        L.orc_scene_set_joint_bounds.argtypes = [P, C.c_int, P, P, C.c_int]
         L.orc_sincosf.argtypes = [C.c_float, P, P]
         L.orc_fk64.argtypes = [P, P, P, P]
         L.orc_fk32.argtypes = [P, P, P]
@@ -146,6 +147,16 @@ class OracleScene:
         ns, nb, nc, ncap = scene.counts
         self.h = lib().orc_scene_create(_p(scene.spheres), ns, _p(scene.boxes), nb, _p(scene.cylinders), nc,
                                         _p(scene.capsules), ncap)
+        jb = getattr(scene, "joint_bounds", None)
+        if isinstance(jb, str):
+            if jb != "none":
+                raise ValueError("joint bounds: None, 'none' or (lower, upper)")
+            lib().orc_scene_set_joint_bounds(self.h, 1, None, None, 0)
+        elif jb is not None:
+            lo = np.ascontiguousarray(jb[0], np.float32).reshape(-1)
+            hi = np.ascontiguousarray(jb[1], np.float32).reshape(-1)
+            self._keep += [lo, hi]
+            lib().orc_scene_set_joint_bounds(self.h, 2, _p(lo), _p(hi), len(lo))
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -59,6 +59,8 @@ struct orc_scene {
     float* box;  /* [nb][18] inverse pose m0..m11, min3, max3 */
     float* cyl;  /* [nc][14] inverse pose m0..m11, radius, half height */
     float* cap;  /* [ncap][8] p1(3), v(3), rdv, radius */
+    int has_bnd; /* 0: the robot's joint limits; 1: bnd below */
+    float bnd[32]; /* lower[16], upper[16] over the active joints (+-inf: unbounded) */
 };
 
 /* ----------------------------------------------------------------------------------------- */
@@ -280,6 +282,15 @@ orc_scene* orc_scene_create(const float* spheres, int ns, const float* boxes, in
         o[7] = c[6];
     }
     return s;
+}
+
+void orc_scene_set_joint_bounds(orc_scene* s, int mode, const float* lower, const float* upper, int n) {
+    s->has_bnd = mode != 0;
+    for (int j = 0; j < 16; j++) {
+        int given = mode == 2 && j < n;
+        s->bnd[j] = given ? lower[j] : -INFINITY;
+        s->bnd[16 + j] = given ? upper[j] : INFINITY;
+    }
 }
 
 void orc_scene_destroy(orc_scene* s) {
@@ -512,11 +523,14 @@ static inline float cap_term(const float* c, float r, const float* k) {
     return d2 < rr ? fmaxf(0.0f, rs - sqrtf(d2)) : 0.0f;
 }
 
-static int bounds_ok(const orc_robot* r, const float* q, float* cost) {
-    /* BoundaryConstraint.cu:56-92 with tolerance 1e-4 (BoundaryConstraint.h:13) */
+static int bounds_ok(const orc_robot* r, const orc_scene* sc, const float* q, float* cost) {
+    /* BoundaryConstraint.cu:56-92 with tolerance 1e-4 (BoundaryConstraint.h:13); the bounds are
+     * the scene's when it carries its own (a space's BoundaryConstraint, or none) */
     float acc = 0.0f;
     for (int j = 0; j < r->dof; j++) {
-        float v = q[j], lo = r->act_lo[j], hi = r->act_hi[j];
+        float v = q[j];
+        float lo = sc && sc->has_bnd ? sc->bnd[j] : r->act_lo[j];
+        float hi = sc && sc->has_bnd ? sc->bnd[16 + j] : r->act_hi[j];
         if (v < lo) { float d = lo - v; acc = fmaf(d, d, acc); }
         else if (v > hi) { float d = v - hi; acc = fmaf(d, d, acc); }
     }
@@ -557,7 +571,7 @@ static int check_one_cost(const orc_robot* r, const orc_scene* sc, const float* 
         part += self_term(dot3f(dx, dx, dy, dy, dz, dz), r->pthr[p], r->psr[p]);
     }
     pk[4] = part;
-    bounds_ok(r, q, &pk[5]);
+    bounds_ok(r, sc, q, &pk[5]);
     float tot = 0.0f;
     for (int k = 0; k < ORC_COST_KINDS; k++) tot += pk[k];
     if (parts) memcpy(parts, pk, sizeof(pk));
@@ -579,7 +593,7 @@ void orc_constraint_costs(const orc_robot* r, const orc_scene* s, const float* q
 /* Early-out verdict in canonical order; returns 1 if valid; *flops = canonical work. */
 static int check_one_canon(const orc_robot* r, const orc_scene* sc, const float* q, const float* cen, double* flops) {
     double f = r->fk_flops + 18.0 * r->S;
-    int valid = bounds_ok(r, q, NULL);
+    int valid = bounds_ok(r, sc, q, NULL);
     if (valid) {
         for (int p = 0; p < r->P && valid; p++) {
             const float* a = cen + 3 * r->pi_[p];

@@ -57,6 +57,9 @@ double orc_full_flops(const orc_robot* r, const orc_scene* s);
  * capsules [ncap][7] = p1(3), p2(3), radius. */
 orc_scene* orc_scene_create(const float* spheres, int ns, const float* boxes, int nb,
                             const float* cylinders, int nc, const float* capsules, int ncap);
+/* Joint bounds of the scene's checks (mp_scene_set_joint_bounds): mode 0 the robot's limits,
+ * 1 none, 2 lower[n] / upper[n] over the active joints (joints from n on unbounded). */
+void orc_scene_set_joint_bounds(orc_scene* s, int mode, const float* lower, const float* upper, int n);
 void orc_scene_destroy(orc_scene* s);
 
 void orc_sincosf(float x, float* s, float* c);

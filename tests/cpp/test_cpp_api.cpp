@@ -173,6 +173,33 @@ int main(int argc, char** argv) {
     std::vector<bool> feas;
     space->checkStates(states, feas);
     std::printf("space_feasible %d %d\n", (int)feas[0], (int)feas[1]);
+    // joint bounds are checked only through a BoundaryConstraint, with its own bounds: a state
+    // beyond the last active joint's limit by 0.5 is feasible in a space without one; a narrow
+    // BoundaryConstraint (that joint within +-0.1) rejects a state at 0.3 and keeps the start
+    {
+        std::vector<float> outside = start;
+        outside[dof - 1] = 1.5f;
+        std::vector<CUDAMPLib::BaseConstraintPtr> cons_nb{env, self};
+        auto space_nb = std::make_shared<CUDAMPLib::SingleArmSpace>(dof, cons_nb, jt, poses, axes, par, smap, spos, rad,
+                                                                    act, lo, hi, def, names, 0.02f);
+        std::vector<bool> f_nb, f_b, f_n;
+        space_nb->checkStates(space_nb->createStatesFromVector({outside}), f_nb);
+        space->checkStates(space->createStatesFromVector({outside}), f_b);
+        std::vector<float> nlo = lo, nhi = hi;
+        for (int i = 0, a = 0; i < L; i++)
+            if (act[i] && a++ == (int)dof - 1) {
+                nlo[i] = -0.1f;
+                nhi[i] = 0.1f;
+            }
+        auto narrow = std::make_shared<CUDAMPLib::BoundaryConstraint>("bound", nlo, nhi, act);
+        std::vector<CUDAMPLib::BaseConstraintPtr> cons_n{env, self, narrow};
+        auto space_n = std::make_shared<CUDAMPLib::SingleArmSpace>(dof, cons_n, jt, poses, axes, par, smap, spos, rad,
+                                                                   act, lo, hi, def, names, 0.02f);
+        std::vector<float> z03 = start;
+        z03[dof - 1] = 0.3f;
+        space_n->checkStates(space_n->createStatesFromVector({start, z03}), f_n);
+        std::printf("space_bounds %d %d %d %d\n", (int)f_nb[0], (int)f_b[0], (int)f_n[0], (int)f_n[1]);
+    }
     std::vector<bool> mf;
     std::vector<float> mc;
     space->checkMotions(space->createStatesFromVector({start}), space->createStatesFromVector({goal}), mf, mc);

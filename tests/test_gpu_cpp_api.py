@@ -51,6 +51,8 @@ def test_cpp_api(tmp_path):
     assert res["cprrtc_after_overflow"] == ["1"]
     assert "exceeds the maximum number of obstacles" in out.stderr
     assert res["space_feasible"] == ["1", "1"]
+    # joint bounds only through a BoundaryConstraint, with its own bounds
+    assert res["space_bounds"] == ["1", "0", "1", "0"]
     assert res["space_motion"][0] == "0" and abs(float(res["space_motion"][1]) - 1.0) < 1e-6
     assert res["space_samples"][0] == "64"
     assert res["space_source"] == ["1", "1"]

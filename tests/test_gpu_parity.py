@@ -252,6 +252,64 @@ def test_self_and_env_rows_point_robot():
     assert list(v) == [0, 1, 0] and (parts[:, 0] > 0).sum() == 3
 
 
+@pytest.mark.parametrize("mode", ["robot", "none", "custom"])
+def test_scene_joint_bounds_identical(mode, fr3, orc_fr3, gpu_robots, scenes, rng):
+    """Joint bounds carried by the scene (the robot's limits, none, or a BoundaryConstraint's
+    own): configurations drawn 30% past the limits on every side; verdicts of the early-out and
+    full kernels, total costs, the per-constraint rows and edge checks equal the oracle's."""
+    lo, hi = fr3.active_lower.astype(np.float64), fr3.active_upper.astype(np.float64)
+    span = hi - lo
+    q = rng.uniform(lo - 0.3 * span, hi + 0.3 * span, size=(4099, fr3.dof)).astype(np.float32)
+    mid = (lo + hi) / 2
+    jb = {"robot": None, "none": "none",
+          "custom": ((mid - 0.25 * span).astype(np.float32), (mid + 0.25 * span).astype(np.float32))}[mode]
+    base = scenes["fr3"]
+    sc = Scene(spheres=base.spheres, boxes=base.boxes, cylinders=base.cylinders, capsules=base.capsules,
+               joint_bounds=jb)
+    ns, so = native.Scene(sc), oracle.OracleScene(sc)
+    r = gpu_robots["fr3"]
+    v = native.check_states(r, ns, q)
+    cost, vc = native.state_costs(r, ns, q)
+    parts, tot, vp = native.constraint_costs(r, ns, q)
+    vo, co, _, _ = oracle.check(orc_fr3, so, q, want_cost=True, nthreads=8)
+    assert np.array_equal(v, vo) and np.array_equal(vc, vo) and np.array_equal(vp, vo)
+    assert np.array_equal(cost, co) and np.array_equal(tot, co)
+    assert np.array_equal(parts, oracle.constraint_costs(orc_fr3, so, q))
+    out = (q < lo) | (q > hi)
+    if mode == "none":
+        assert not parts[5].any()
+    else:
+        assert (parts[5] > 0).sum() > 1000
+    # edges between consecutive configurations
+    ge, gl = native.check_edges(r, ns, q[:-1:4], q[1::4], 0.02)
+    oe, ol = oracle.edges(orc_fr3, so, q[:-1:4], q[1::4], 0.02)
+    assert np.array_equal(ge, oe) and np.array_equal(gl, ol)
+    assert out.any()
+
+
+def test_planner_follows_scene_joint_bounds(fr3, gpu_robots):
+    """A start past the robot's joint limit is invalid under the robot's limits and plannable
+    without a joint-bound check (CPRRTC's kernel checks none)."""
+    r = gpu_robots["fr3"]
+    orc = oracle.OracleRobot(fr3)
+    free = oracle.OracleScene(Scene(joint_bounds="none"))
+    rng = np.random.default_rng(5)
+    for _ in range(1000):  # a start 0.2 past joint 4's upper limit and a goal inside, both free
+        start = orc.sample(seed=int(rng.integers(1 << 30)), idx0=0, n=2)
+        start, goal = start[0].copy(), start[1].copy()
+        start[3] = fr3.active_upper[3] + np.float32(0.2)
+        if oracle.check(orc, free, np.stack([start, goal]))[0].all():
+            break
+    res = {}
+    for jb in (None, "none"):
+        res[jb] = native.rrtc_solve(r, native.Scene(Scene(joint_bounds=jb)), start, goal, native.default_params(seed=1))
+    assert res[None]["status"] == "invalid"
+    assert res["none"]["status"] == "solved"
+    p = res["none"]["path"]
+    assert np.array_equal(p[0], start) and np.array_equal(p[-1], goal)
+    assert oracle.validate_path(orc, free, p, 0.005)[0]
+
+
 def test_c2_full_size_against_oracle(fr3, orc_fr3, gpu_robots, c2):
     """All 2^20 C2 verdicts and sphere centres against the oracle, plus size-independent
     properties: idempotence and chunked == whole-batch evaluation."""

@@ -254,6 +254,40 @@ def test_joint_bounds_tolerance():
     assert cost[2] == pytest.approx(0.011, rel=1e-3)
 
 
+def test_scene_joint_bounds_modes():
+    """A scene carries the joint bounds of its checks: the robot's limits by default, none (a
+    space without a BoundaryConstraint), or custom bounds over the active joints (a
+    BoundaryConstraint's own), joints past the given ones unbounded.  Costs: the square root of
+    the summed squared violations, zero below 1e-4 on the sum."""
+    m = point_robot(0.01, lim=1.0)
+    r = oracle.OracleRobot(m)
+    q = np.array([[1.5, 0, 0], [0.8, -0.9, 0.0], [0.0, 0.0, 5.0], [0.505, 0, 0], [0.2, 0.1, -0.3]], np.float32)
+
+    def run(jb):
+        s = oracle.OracleScene(Scene(joint_bounds=jb))
+        v, cost, _, _ = oracle.check(r, s, q, want_cost=True)
+        parts = oracle.constraint_costs(r, s, q)
+        assert np.array_equal(parts[5], cost) and not parts[:5].any()
+        return list(v), cost
+
+    v, cost = run(None)  # the robot's limits, +-1
+    assert v == [0, 1, 0, 1, 1]
+    assert cost[0] == pytest.approx(0.5, rel=1e-6) and cost[2] == pytest.approx(4.0, rel=1e-6)
+    v, cost = run("none")
+    assert v == [1, 1, 1, 1, 1] and not cost.any()
+    v, cost = run(([-0.5, -0.5, -0.5], [0.5, 0.5, 0.5]))
+    # violations 1.0; 0.3 and 0.4; 4.5; 0.005 (squared 2.5e-5, under the tolerance); none
+    assert v == [0, 0, 0, 1, 1]
+    assert cost[0] == pytest.approx(1.0, rel=1e-6)
+    assert cost[1] == pytest.approx(0.5, rel=1e-6)
+    assert cost[2] == pytest.approx(4.5, rel=1e-6)
+    # two joints given: the third is unbounded; -inf / +inf open a side
+    v, cost = run(([-0.5, -np.inf], [0.5, 0.0]))
+    assert v == [0, 0, 1, 1, 0]
+    assert cost[0] == pytest.approx(1.0, rel=1e-6) and cost[1] == pytest.approx(0.3, rel=1e-6)
+    assert cost[4] == pytest.approx(0.1, rel=1e-5)
+
+
 def test_self_pairs_match_srdf_rule(fr3, fetch, orc_fr3, orc_fetch):
     assert orc_fr3.P == 754 and orc_fetch.P == 2534
     i, j, thr = orc_fr3.pairs()
