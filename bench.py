@@ -212,6 +212,91 @@ def bench_mbm(models, sampler=0, dynamic_domain=0, balance=0):
     return out, paths
 
 
+RRG_SETTINGS = {"samples_per_iteration": 60, "max_travel_distance": 6.0, "resolution": 0.02}
+
+
+def rrg_tasks(robot, model, n, seed):
+    """evaluate_cudampl.cpp's RRG workload: FR3 with self collision and 20 sphere obstacles
+    (radius U[0.03, 0.1], centres in [-0.8, 0.8]^2 x [0, 1.2] clear of the base column) per task,
+    a collision-free start and goal whose straight line is blocked."""
+    rng = np.random.default_rng(seed)
+    tasks = []
+    while len(tasks) < n:
+        c = rng.uniform([-0.8, -0.8, 0.0], [0.8, 0.8, 1.2], size=(20, 3))
+        c = c[np.hypot(c[:, 0], c[:, 1]) > 0.3]
+        r = rng.uniform(0.03, 0.1, size=len(c))
+        scene = workloads.Scene(spheres=np.concatenate([c, r[:, None]], axis=1).astype(np.float32))
+        ns = native.Scene(scene)
+        pair = workloads.sample_pair(model, scene, rng, lambda m, sc, q: native.check_states(robot, ns, q),
+                                     lambda m, sc, a, b: native.check_edges(robot, ns, a, b, 0.02)[0],
+                                     want_free_line=False)
+        if pair is not None:
+            tasks.append((scene, ns, pair))
+    return tasks
+
+
+def roadmap_path(g):
+    """Shortest start-goal path over the roadmap's edge lengths (RRG.cpp:433-491), or None."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import dijkstra
+    nv = g["nv"]
+    if not g["solved"] or g["ne"] == 0:
+        return None
+    e, w = g["edges"], g["weights"].astype(np.float64)
+    m = coo_matrix((np.concatenate([w, w]), (np.concatenate([e[:, 0], e[:, 1]]), np.concatenate([e[:, 1], e[:, 0]]))),
+                   shape=(nv, nv)).tocsr()
+    dist, pred = dijkstra(m, indices=0, return_predecessors=True)
+    if not np.isfinite(dist[1]):
+        return None
+    path, v = [1], 1
+    while v != 0:
+        v = int(pred[v])
+        path.append(v)
+    return g["vertices"][path[::-1]]
+
+
+def bench_rrg(robot, model, n_tasks, max_passes=2000):
+    """The roadmap planner (RRG::solve's loop, RRG.cpp:241-424) with the roadmap in device memory,
+    evaluate_cudampl.cpp's settings (travel distance 6.0, 60 samples per pass, resolution 0.02):
+    per task, the time from the roadmap's creation to the graph read back, and Dijkstra's path."""
+    tasks = rrg_tasks(robot, model, n_tasks, seed=11)
+    times, passes, verts, paths, solved = [], [], [], [], 0
+    for k, (scene, ns, (s, g)) in enumerate(tasks):
+        native.sync()
+        t0 = time.perf_counter()
+        rm = native.Roadmap(robot, ns, s[None], g[None], seed=k, **RRG_SETTINGS)
+        for _ in range(max_passes):
+            if rm.iterate():
+                break
+        graph = rm.read()
+        times.append((time.perf_counter() - t0) * 1e3)
+        rm.close()
+        passes.append(graph["passes"])
+        verts.append(graph["nv"])
+        path = roadmap_path(graph)
+        if path is not None:
+            solved += 1
+            paths.append((model, scene, path, s, g))
+    return ({"tasks": n_tasks, "solved": solved, "success_rate": solved / n_tasks, "settings": RRG_SETTINGS,
+             "p50_ms": float(np.percentile(times, 50)), "p99_ms": float(np.percentile(times, 99)),
+             "mean_ms": float(np.mean(times)), "passes_p50_max": [float(np.median(passes)), int(max(passes))],
+             "vertices_p50_max": [float(np.median(verts)), int(max(verts))]}, paths, tasks)
+
+
+def cpu_rrg(model, tasks, n, max_passes=2000):
+    """The oracle's roadmap loop (the same passes, bit-identical graphs) on the first n tasks."""
+    import oracle
+    orc = oracle.OracleRobot(model)
+    ms = []
+    for k, (scene, _, (s, g)) in enumerate(tasks[:n]):
+        t0 = time.perf_counter()
+        oracle.rrg(orc, oracle.OracleScene(scene), s[None], g[None], RRG_SETTINGS["samples_per_iteration"],
+                   RRG_SETTINGS["max_travel_distance"], RRG_SETTINGS["resolution"], k, 0, max_passes)
+        ms.append((time.perf_counter() - t0) * 1e3)
+    return {"tasks": n, "p50_ms": float(np.percentile(ms, 50)), "mean_ms": float(np.mean(ms)), "cores": 1,
+            "kind": "port"}
+
+
 def serve(robots, lib, rate, seconds, seed, sample_paths=1000, saturation_queries=0):
     """Served planning over a scene library (workloads.c5_stream): Poisson arrivals at `rate`
     (or, with saturation_queries, that many queries submitted at once), scenes reused with
@@ -391,6 +476,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--c3-seeds", type=int, default=1000)
     ap.add_argument("--c1-seeds", type=int, default=100)
+    ap.add_argument("--rrg-tasks", type=int, default=32)
     ap.add_argument("--tune-queries", type=int, default=256)
     ap.add_argument("--team-waves", type=int, default=0, help="planning launch shape (0: tuned / default)")
     ap.add_argument("--waves", type=int, default=0, help="planning grid in waves (0: tuned / resident)")
@@ -403,7 +489,7 @@ def main():
     ap.add_argument("--balance", choices=["smaller", "split", "dynamic", "alternate"], default="smaller",
                     help="tree balance of the batched planner (CPRRTC's smaller tree, or pRRTC's modes)")
     ap.add_argument("--skip", nargs="*", default=[],
-                    help="parts to skip: tune plain serve c2 c3 mbm cpu (cpu also skips c1 and the CPU re-check)")
+                    help="parts to skip: tune plain serve c2 c3 mbm rrg cpu (cpu also skips c1 and the CPU re-check)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: queries shard across GPUs with no exchange, so the "
@@ -601,6 +687,11 @@ def main():
         out["mbm"], checks["mbm"] = bench_mbm(models, params.sampler, params.dynamic_domain, params.balance)
         log("mbm:", out["mbm"])
 
+    rrg_tasks_ = None
+    if "rrg" not in a.skip and a.rrg_tasks > 0:
+        out["rrg"], checks["rrg"], rrg_tasks_ = bench_rrg(robots["fr3"], models[0], a.rrg_tasks)
+        log("rrg:", out["rrg"])
+
     if "cpu" not in a.skip:
         host = host_info()
         out["host"] = host
@@ -618,6 +709,10 @@ def main():
             out["batched"]["valid_solved_qps"] = batched_qps * v["rate"]
         out["cpu_checker_c2"] = cpu_checker_c2(models[0], threads)
         log("cpu c2:", out["cpu_checker_c2"])
+        if rrg_tasks_:
+            out["rrg"]["cpu_port"] = cpu_rrg(models[0], rrg_tasks_, min(8, len(rrg_tasks_)))
+            out["rrg"]["speedup_p50"] = out["rrg"]["cpu_port"]["p50_ms"] / out["rrg"]["p50_ms"]
+            log("cpu rrg:", out["rrg"]["cpu_port"])
         if "c2" in out:  # C2's early-out throughput at the canonical-order work count
             cf_ = out["cpu_checker_c2"]["canonical_flops_per_config"]
             v = out["c2"]["verdict"]
