@@ -522,7 +522,7 @@ def main():
         c_g = [q[3] for q in cal]
         for name, per_query, n in (("batch", False, len(cal)), ("plain", True, min(32, len(cal)))):
             best, cands = native.rrtc_autotune(c_rob[:n], c_sc[:n], c_s[:n], c_g[:n], native.default_params(seed=77),
-                                               per_query=per_query, reps=2 if per_query else 4)
+                                               per_query=per_query, reps=2 if per_query else 12)
             tune[name] = {"team_waves": best.team_waves, "waves": best.waves, "teams_per_query": best.teams_per_query,
                           "lanes_per_iteration": best.lanes_per_iteration, "calibration_queries": n,
                           "candidates": cands}

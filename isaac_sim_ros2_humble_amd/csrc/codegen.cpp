@@ -465,6 +465,34 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
             X.os << "        if (__ballot(" << X.ref(d2) << " < " << lit(cull) << ")) {\n";
             int nsp = (int)(A.spheres.size() + B.spheres.size());
             X.os << "            work += " << (int)lp.pairs.size() * kSelf + nsp * kXform << "u;\n";
+            if (!cost) {
+                // verdict: hold the centres of the link with fewer spheres and stream the other
+                // link's spheres one at a time (the pair set is every sphere of A with every
+                // sphere of B; OR-ing the contacts does not depend on their order)
+                const bool a_small = A.spheres.size() <= B.spheres.size();
+                const SphereLink& Hk = a_small ? A : B;
+                const SphereLink& Sk = a_small ? B : A;
+                std::map<int, std::array<V, 3>> held;
+                for (int s : Hk.spheres) { V c[3]; centre(X, s, c); held[s] = {c[0], c[1], c[2]}; }
+                for (int s : Sk.spheres) {
+                    V c[3];
+                    centre(X, s, c);
+                    for (int hsph : Hk.spheres) {
+                        float thr, sr;
+                        pair_thr(std::min(hsph, s), std::max(hsph, s), thr, sr);
+                        V e[3];
+                        const int i = std::min(hsph, s);
+                        const std::array<V, 3>& ci = i == s ? std::array<V, 3>{c[0], c[1], c[2]} : held[hsph];
+                        const std::array<V, 3>& cj = i == s ? held[hsph] : std::array<V, 3>{c[0], c[1], c[2]};
+                        for (int a = 0; a < 3; a++) e[a] = X.sub(ci[a], cj[a]);
+                        V dd = X.dot3(e[0], e[0], e[1], e[1], e[2], e[2]);
+                        X.os << "            h |= " << X.ref(dd) << " < " << lit(std::min(thr, sqrt_threshold(sr))) << ";\n";
+                    }
+                }
+                X.os << "        }\n        }\n";
+                X.os << "        if (mp_stop<MODE>(h, mask)) return h;\n        }\n";
+                continue;
+            }
             std::map<int, std::array<V, 3>> cs;
             for (int s : A.spheres) { V c[3]; centre(X, s, c); cs[s] = {c[0], c[1], c[2]}; }
             for (int s : B.spheres) { V c[3]; centre(X, s, c); cs[s] = {c[0], c[1], c[2]}; }

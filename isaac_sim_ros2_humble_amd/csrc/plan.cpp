@@ -499,28 +499,34 @@ int mp_rrtc_autotune(int nq, mp_robot* const* robots, const mp_scene* const* sce
         int n = 0;
         Cand best_c{0, 0, 0, 0};
         double bms = 1e300;
-        for (const Cand& cd : cands) {
-            mp_rrtc_params p = *params;
-            p.team_waves = cd.team;
-            p.waves = cd.waves;
-            p.teams_per_query = cd.cap;
-            p.lanes_per_iteration = cd.lanes;
-            double ms = 0.0;
-            for (int r = 0; r < reps; r++) {
+        // every candidate once per round, rounds = reps (one seed each), so that drifts of the
+        // device over the tuning run fall on all candidates alike; the score is the mean time
+        std::vector<double> total(cands.size(), 0.0);
+        for (int r = 0; r < reps; r++) {
+            for (size_t c = 0; c < cands.size(); c++) {
+                const Cand& cd = cands[c];
+                mp_rrtc_params p = *params;
+                p.team_waves = cd.team;
+                p.waves = cd.waves;
+                p.teams_per_query = cd.cap;
+                p.lanes_per_iteration = cd.lanes;
                 p.seed = params->seed + 7919ull * (uint64_t)r;
                 for (auto& o : out) std::memset(&o, 0, sizeof(o));
                 if (per_query) {
                     for (int k = 0; k < nq; k++) {
                         mp::solve_impl("plain", 1, robots + k, scenes + k, starts16 + 16 * k, goals16 + 16 * k, &p,
                                        &out[k], 512);
-                        ms += mp_last_kernel_ms("plan");
+                        total[c] += mp_last_kernel_ms("plan");
                     }
                 } else {
                     mp::solve_impl("batch", nq, robots, scenes, starts16, goals16, &p, out.data(), 0);
-                    ms += mp_last_kernel_ms("plan");
+                    total[c] += mp_last_kernel_ms("plan");
                 }
             }
-            ms /= reps;
+        }
+        for (size_t c = 0; c < cands.size(); c++) {
+            const Cand& cd = cands[c];
+            const double ms = total[c] / reps;
             if (results && n < cap) results[n] = mp_tune_result{cd.team, cd.waves, cd.cap, cd.lanes, ms};
             n++;
             if (ms < bms) {
