@@ -668,6 +668,25 @@ def main():
             if "plain" in out:
                 out["served_vs_plain"] = r["throughput_qps"] / out["plain"]["qps"]
             log("served saturation: %s" % {k: v for k, v in r.items() if k != "statuses"})
+            if "plain" not in a.skip:
+                # the plain planner on the served workload itself: the first 256 queries of the
+                # saturation stream, one launch each, scenes resident beforehand
+                first = [(s_, p_) for _, s_, p_, _ in workloads.c5_stream(lib, 20000.0, a.saturation_queries / 20000.0,
+                                                                          p_reuse=0.8, seed=99)][:256]
+                dscenes = {s_: native.Scene(lib[s_][1]) for s_, _ in first}
+                native.sync()
+                t0 = time.perf_counter()
+                nsol = 0
+                for k, (s_, p_) in enumerate(first):
+                    st_, gl_ = lib[s_][2][p_]
+                    res_ = native.rrtc_solve(robots[models[lib[s_][0]].name], dscenes[s_], st_, gl_,
+                                             native.default_params(seed=k))
+                    nsol += res_["status"] == "solved"
+                native.sync()
+                pq = nsol / (time.perf_counter() - t0)
+                out["plain_served_workload"] = {"qps": pq, "queries": len(first), "solved": nsol}
+                out["served_vs_plain_same_workload"] = r["throughput_qps"] / pq if pq > 0 else None
+                log("plain on the served workload: %.1f q/s" % pq)
         checks["served"] = served_checks
 
     if "c2" not in a.skip:
