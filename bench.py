@@ -325,7 +325,10 @@ def serve(robots, lib, rate, seconds, seed, sample_paths=1000, saturation_querie
         a, b = lib[s][2][p]
         tickets.append((srv.submit(lib[s][0], dev[s], a, b, seed=len(tickets)), s, p))
     submit_s = time.perf_counter() - t0
+    at_submit = srv.stats()
+    t_sub = time.perf_counter()
     srv.wait_idle(600.0)
+    t_idle = time.perf_counter()
     st = srv.stats()
     statuses, paths = {}, []
     every = max(1, len(tickets) // sample_paths)
@@ -340,7 +343,16 @@ def serve(robots, lib, rate, seconds, seed, sample_paths=1000, saturation_querie
            "success_rate": st["solved"] / max(st["completed"], 1), "scene_uploads": uploads,
            "throughput_qps": st["throughput_qps"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
            "mean_ms": st["mean_ms"], "steps": st["steps"], "worker_step_ms": st["step_ms"],
-           "worker_loop_ms": st["loop_ms"], "statuses": statuses, "submit_seconds": submit_s}
+           "worker_loop_ms": st["loop_ms"], "backlog_steps": st["backlog_steps"], "statuses": statuses,
+           "submit_seconds": submit_s}
+    if saturation_queries:
+        # after the last submit every query is queued or running: the completions from there to
+        # idle are the server's rate with a full backlog (the submit loop itself, ~13 us per
+        # call from Python, caps the arrival rate before that)
+        drained = st["completed"] - at_submit["completed"]
+        out["backlog_at_last_submit"] = at_submit["queued"] + at_submit["active"]
+        out["drain_qps"] = drained / max(t_idle - t_sub, 1e-9)
+        out["drain_queries"] = drained
     return out, paths
 
 
@@ -686,6 +698,7 @@ def main():
                 pq = nsol / (time.perf_counter() - t0)
                 out["plain_served_workload"] = {"qps": pq, "queries": len(first), "solved": nsol}
                 out["served_vs_plain_same_workload"] = r["throughput_qps"] / pq if pq > 0 else None
+                out["served_drain_vs_plain_same_workload"] = r["drain_qps"] / pq if pq > 0 else None
                 log("plain on the served workload: %.1f q/s" % pq)
         checks["served"] = served_checks
 

@@ -407,7 +407,8 @@ typedef struct {
     size_t tree_memory_bytes;  /* device memory for tree buffers; admission is by free memory */
     int tree_capacity;         /* nodes per tree */
     int max_active;            /* upper bound on concurrently active queries */
-    int step_iterations;       /* extend-connect iterations per step launch (0 = auto) */
+    int step_iterations;       /* team passes per step launch at least (0 = 2 per team); a step
+                                * gives every active query 32 passes when that is more */
     int waves;                 /* grid size in waves (0 = auto) */
     int use_graph;             /* capture the step in a hipGraph and replay it */
     mp_rrtc_params params;
@@ -420,6 +421,7 @@ typedef struct {
     int64_t steps;
     double step_ms;  /* worker time inside steps: enqueue, device work, synchronisation */
     double loop_ms;  /* worker time in whole steps: admission, the step, retiring finished queries */
+    int64_t backlog_steps;  /* steps that ran while queries waited for a slot */
 } mp_server_stats;
 
 /* A server thread owns the device; submit/poll are thread-safe. */

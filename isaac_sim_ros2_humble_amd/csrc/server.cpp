@@ -87,22 +87,32 @@ struct mp_server {
     // worker-thread time in step() (enqueue and device) and in whole steps (admission, step,
     // retiring); written by the worker only
     std::atomic<double> busy_ms{0.0}, loop_ms{0.0};
+    std::atomic<int64_t> backlog_steps{0};  // steps that ran with queries waiting for a slot
 
     void run();
-    void step(const std::vector<int>& list, int n_new, bool backlog);
+    void step(const std::vector<int>& list, int n_new);
 };
 
-// Queries waiting for a slot (backlog): the step runs kBacklogBudget times the iterations, so
-// the step's fixed costs (launch, copies, synchronisation, retiring) are spread over more work;
-// without a backlog short steps keep the latency low.
-constexpr int kBacklogBudget = 4;
+// A step's ticket budget (one team pass per ticket) grows with the number of active queries:
+// kTicketsPerActive per query, at least the base budget.  With few queries active a step stays
+// short (a new query waits at most one short step for admission); with many, each step gives
+// every query a few passes, so the step's fixed costs (launch, copies, synchronisation, the
+// wait for the last pass, retiring) are spread over more work.  Measured on the served
+// workload at saturation: 4 / 8 / 16 / 32 / 64 tickets per active query gave 51.7 / 54.1 /
+// 65.8 / 77.4 / 87.4 K queries/s with p50 2.1 / 1.8 / 1.9 / 3.0 / 22.6 ms; the latencies at
+// 1000 and 5000 queries/s did not change.
+constexpr int kTicketsPerActive = 32;
 
-void mp_server::step(const std::vector<int>& list, int n_new, bool backlog) {
+void mp_server::step(const std::vector<int>& list, int n_new) {
     int* h = pool->hpinned;
     h[0] = 0;
     h[1] = (int)list.size();
     h[2] = (int)list.size();
-    h[3] = backlog ? kBacklogBudget * budget : budget;
+    // (exclusive mode counts the budget per team, one query each: it stays as configured)
+    h[3] = prm.params.exclusive
+               ? budget
+               : (int)std::min<long long>(std::max<long long>(budget, (long long)kTicketsPerActive * (long long)list.size()),
+                                          1 << 30);
     h[4] = (int)list.size() - n_new;
     h[5] = n_new;
     h[6] = prm.params.exclusive ? 1 : 0;
@@ -187,7 +197,8 @@ void mp_server::run() {
             }
             if (list.empty()) continue;
             const double t0 = now_ms();
-            step(list, (int)fresh.size(), backlog);
+            step(list, (int)fresh.size());
+            if (backlog) backlog_steps.fetch_add(1, std::memory_order_relaxed);
             const double t1 = now_ms();
             busy_ms.store(busy_ms.load(std::memory_order_relaxed) + (t1 - t0), std::memory_order_relaxed);
             steps++;
@@ -390,6 +401,7 @@ int mp_server_stats_get(mp_server* s, mp_server_stats* out) {
         out->steps = s->steps;
         out->step_ms = s->busy_ms.load(std::memory_order_relaxed);
         out->loop_ms = s->loop_ms.load(std::memory_order_relaxed);
+        out->backlog_steps = s->backlog_steps.load(std::memory_order_relaxed);
         if (!s->latencies.empty()) {
             std::vector<double> v = s->latencies;
             std::sort(v.begin(), v.end());

@@ -73,7 +73,7 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("solved", C.c_int64), ("active", C.c_int),
                 ("queued", C.c_int), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
                 ("throughput_qps", C.c_double), ("steps", C.c_int64), ("step_ms", C.c_double),
-                ("loop_ms", C.c_double)]
+                ("loop_ms", C.c_double), ("backlog_steps", C.c_int64)]
 
 
 # name -> (restype, argtypes)
