@@ -12,11 +12,15 @@ class BaseStates {
     bool isValid() const { return valid_; }
     // total cost per state after checkStates(states) (BaseStates::getTotalCostsHost)
     std::vector<float> getTotalCostsHost() const { return total_costs_; }
+    // the cost of each constraint of the space, in the space's constraint order, per state:
+    // [state][constraint] (BaseStates::getCostsHost, States.cu:65-78), after checkStates(states)
+    std::vector<std::vector<float>> getCostsHost() const { return costs_; }
 
   protected:
     int num_of_states_ = 0;
     bool valid_ = true;
     std::vector<float> total_costs_;
+    std::vector<std::vector<float>> costs_;
     friend class SingleArmSpace;
 };
 typedef std::shared_ptr<BaseStates> BaseStatesPtr;

@@ -67,6 +67,10 @@ class SingleArmSpace : public BaseSpace {
     unsigned long long seed_ = 0x5eed;
     mp_robot* robot_ = nullptr;
     mp_scene* scene_ = nullptr;
+    // per constraint (in the order given): a scene holding only that constraint (its obstacles,
+    // or its joint bounds) and the cost row of mp_constraint_costs that is its cost
+    std::vector<mp_scene*> con_scenes_;
+    std::vector<int> con_rows_;
 };
 typedef std::shared_ptr<SingleArmSpace> SingleArmSpacePtr;
 

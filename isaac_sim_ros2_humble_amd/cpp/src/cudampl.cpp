@@ -217,6 +217,14 @@ SingleArmSpace::SingleArmSpace(size_t dim, const std::vector<BaseConstraintPtr>&
                  "mp_scene_create");
     // joint bounds are checked only through a BoundaryConstraint of the space, with that
     // constraint's own bounds (SingleArmSpace.cu:1572-1602, BoundaryConstraint.cu:56-92)
+    auto active_bounds = [&](const BoundaryConstraint& bc, std::vector<float>& lo, std::vector<float>& hi) {
+        for (size_t j = 0; j < active_joint_map.size(); j++) {
+            if (!active_joint_map[j]) continue;
+            const bool on = j < bc.active_.size() && bc.active_[j] && j < bc.lower_.size() && j < bc.upper_.size();
+            lo.push_back(on ? bc.lower_[j] : -INFINITY);
+            hi.push_back(on ? bc.upper_[j] : INFINITY);
+        }
+    };
     BoundaryConstraintPtr bc;
     for (const auto& con : constraints)
         if (auto p = std::dynamic_pointer_cast<BoundaryConstraint>(con)) bc = p;
@@ -224,18 +232,60 @@ SingleArmSpace::SingleArmSpace(size_t dim, const std::vector<BaseConstraintPtr>&
         mpcpp::check(mp_scene_set_joint_bounds(scene_, MP_BOUNDS_NONE, nullptr, nullptr, 0), "mp_scene_set_joint_bounds");
     } else {
         std::vector<float> lo, hi;
-        for (size_t j = 0; j < active_joint_map.size(); j++) {
-            if (!active_joint_map[j]) continue;
-            const bool on = j < bc->active_.size() && bc->active_[j] && j < bc->lower_.size() && j < bc->upper_.size();
-            lo.push_back(on ? bc->lower_[j] : -INFINITY);
-            hi.push_back(on ? bc->upper_[j] : INFINITY);
-        }
+        active_bounds(*bc, lo, hi);
         mpcpp::check(mp_scene_set_joint_bounds(scene_, MP_BOUNDS_CUSTOM, lo.data(), hi.data(), (int)lo.size()),
                      "mp_scene_set_joint_bounds");
+    }
+    // one scene per constraint for its cost row (getCostsHost): the constraint's own obstacles
+    // or bounds, nothing else
+    for (const auto& con : constraints) {
+        std::vector<float> cs, cb, cc;
+        int row = -1;
+        BoundaryConstraintPtr cbnd;
+        if (auto e = std::dynamic_pointer_cast<EnvConstraintSphere>(con)) {
+            row = MP_COST_ENV_SPHERE;
+            for (size_t i = 0; i < e->pos_.size(); i++) cs.insert(cs.end(), {e->pos_[i][0], e->pos_[i][1], e->pos_[i][2], e->radius_[i]});
+        } else if (auto e = std::dynamic_pointer_cast<EnvConstraintCuboid>(con)) {
+            row = MP_COST_ENV_CUBOID;
+            for (size_t i = 0; i < e->pos_.size(); i++) {
+                cb.insert(cb.end(), e->pos_[i].begin(), e->pos_[i].end());
+                cb.insert(cb.end(), e->rot_[i].begin(), e->rot_[i].end());
+                cb.insert(cb.end(), e->min_[i].begin(), e->min_[i].end());
+                cb.insert(cb.end(), e->max_[i].begin(), e->max_[i].end());
+            }
+        } else if (auto e = std::dynamic_pointer_cast<EnvConstraintCylinder>(con)) {
+            row = MP_COST_ENV_CYLINDER;
+            for (size_t i = 0; i < e->pos_.size(); i++) {
+                cc.insert(cc.end(), e->pos_[i].begin(), e->pos_[i].end());
+                cc.insert(cc.end(), e->rot_[i].begin(), e->rot_[i].end());
+                cc.insert(cc.end(), {e->radius_[i], e->height_[i]});
+            }
+        } else if (std::dynamic_pointer_cast<SelfCollisionConstraint>(con)) {
+            row = MP_COST_SELF;
+        } else if (auto p = std::dynamic_pointer_cast<BoundaryConstraint>(con)) {
+            row = MP_COST_BOUNDS;
+            cbnd = p;
+        }
+        mp_scene* cscene = nullptr;
+        mpcpp::check(mp_scene_create(cs.data(), (int)(cs.size() / 4), cb.data(), (int)(cb.size() / 18), cc.data(),
+                                     (int)(cc.size() / 14), nullptr, 0, &cscene),
+                     "mp_scene_create");
+        con_scenes_.push_back(cscene);
+        con_rows_.push_back(row);
+        if (cbnd) {
+            std::vector<float> lo, hi;
+            active_bounds(*cbnd, lo, hi);
+            mpcpp::check(mp_scene_set_joint_bounds(cscene, MP_BOUNDS_CUSTOM, lo.data(), hi.data(), (int)lo.size()),
+                         "mp_scene_set_joint_bounds");
+        } else {
+            mpcpp::check(mp_scene_set_joint_bounds(cscene, MP_BOUNDS_NONE, nullptr, nullptr, 0), "mp_scene_set_joint_bounds");
+        }
     }
 }
 
 SingleArmSpace::~SingleArmSpace() {
+    for (mp_scene* cs : con_scenes_)
+        if (cs) mp_scene_destroy(cs);
     if (scene_) mp_scene_destroy(scene_);
     if (robot_) mp_robot_destroy(robot_);
 }
@@ -309,11 +359,24 @@ void SingleArmSpace::checkStates(const BaseStatesPtr& states) {
     auto st = as_states(states);
     int n = st->getNumOfStates();
     st->total_costs_.assign(n, 0.0f);
+    st->costs_.assign(n, std::vector<float>(con_rows_.size(), 0.0f));
     if (n == 0) return;
     DevBuf c((size_t)n * sizeof(float));
     mpcpp::check(mp_state_costs(robot_, scene_, st->getJointStatesCuda(), n, (float*)c.p, nullptr, nullptr),
                  "mp_state_costs");
     mpcpp::check(mp_memcpy_d2h(st->total_costs_.data(), c.p, n * sizeof(float), nullptr), "d2h");
+    // each constraint's cost: its row of the per-kind costs on its own scene
+    DevBuf parts((size_t)MP_COST_KINDS * n * sizeof(float));
+    std::vector<float> row(n);
+    for (size_t k = 0; k < con_rows_.size(); k++) {
+        if (con_rows_[k] < 0) continue;
+        mpcpp::check(mp_constraint_costs(robot_, con_scenes_[k], st->getJointStatesCuda(), n, (float*)parts.p, nullptr,
+                                         nullptr, nullptr),
+                     "mp_constraint_costs");
+        mpcpp::check(mp_memcpy_d2h(row.data(), (const float*)parts.p + (size_t)con_rows_[k] * n, n * sizeof(float), nullptr),
+                     "d2h");
+        for (int i = 0; i < n; i++) st->costs_[i][k] = row[i];
+    }
 }
 
 bool SingleArmSpace::checkMotions(const BaseStatesPtr& states1, const BaseStatesPtr& states2, std::vector<bool>& feas,

@@ -173,6 +173,23 @@ int main(int argc, char** argv) {
     std::vector<bool> feas;
     space->checkStates(states, feas);
     std::printf("space_feasible %d %d\n", (int)feas[0], (int)feas[1]);
+    // per-constraint costs (env, self, bound) of the start, a state inside the wall and one past
+    // the last joint's limit by 0.5
+    {
+        std::vector<float> wall = start, beyond = start;
+        for (auto& v : wall) v = 0.0f;
+        beyond[dof - 1] = 1.5f;
+        auto cst = space->createStatesFromVector({start, wall, beyond});
+        space->checkStates(cst);
+        auto costs = cst->getCostsHost();
+        auto total = cst->getTotalCostsHost();
+        std::printf("space_costs");
+        for (size_t i = 0; i < costs.size(); i++) {
+            for (float c : costs[i]) std::printf(" %.9g", c);
+            std::printf(" %.9g", total[i]);
+        }
+        std::printf("\n");
+    }
     // joint bounds are checked only through a BoundaryConstraint, with its own bounds: a state
     // beyond the last active joint's limit by 0.5 is feasible in a space without one; a narrow
     // BoundaryConstraint (that joint within +-0.1) rejects a state at 0.3 and keeps the start

@@ -53,6 +53,15 @@ def test_cpp_api(tmp_path):
     assert res["space_feasible"] == ["1", "1"]
     # joint bounds only through a BoundaryConstraint, with its own bounds
     assert res["space_bounds"] == ["1", "0", "1", "0"]
+    # getCostsHost: per state, the constraints' costs in the space's order (env cuboid, self,
+    # bound) and the total, equal to the oracle's per-kind rows
+    got = np.array([float(x) for x in res["space_costs"]], np.float32).reshape(3, 4)
+    qs = np.array([start, [0.0, 0.0, 0.0], start[:2] + [1.5]], np.float32)
+    parts = oracle.constraint_costs(ro, so, qs)
+    assert np.array_equal(got[:, 0], parts[1]) and np.array_equal(got[:, 1], parts[4])
+    assert np.array_equal(got[:, 2], parts[5])
+    assert got[1, 0] > 0 and got[2, 2] == pytest.approx(0.5, rel=1e-6) and not got[0, :3].any()
+    assert np.array_equal(got[:, 3], oracle.check(ro, so, qs, want_cost=True)[1])
     assert res["space_motion"][0] == "0" and abs(float(res["space_motion"][1]) - 1.0) < 1e-6
     assert res["space_samples"][0] == "64"
     assert res["space_source"] == ["1", "1"]
