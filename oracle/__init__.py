@@ -58,6 +58,7 @@ def lib():
         L.orc_radical_inverse.restype = C.c_float
         L.orc_philox4x32.argtypes = [C.c_uint32] * 6 + [P]
         L.orc_rrtconnect.argtypes = [P, P, P, P, C.c_float, C.c_float, C.c_uint64, C.c_int, C.c_double, P, C.c_int, P]
+        L.orc_cprrtc.argtypes = [P, P, P, P, C.c_float, C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int, P, C.c_int, P]
         _lib = L
     return _lib
 
@@ -270,6 +271,30 @@ def rrg(robot, scene, starts, goals, k, max_travel, resolution, seed, offset, ma
     out.update(vertices=v[:nv].copy(), groups=g[:nv].copy(), edges=e[:ne].copy(), weights=w[:ne].copy(),
                start_members=sm[:info.start_group].copy(), goal_members=gm[:info.goal_group].copy())
     return out
+
+
+class CprrtcInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("start_tree", C.c_int), ("goal_tree", C.c_int),
+                ("path_len", C.c_int)]
+
+
+CPRRTC_STATUS = {1: "solved", 2: "exhausted", 3: "tree_full", 4: "invalid"}
+
+
+def cprrtc(robot, scene, start, goal, resolution=0.02, max_step=32, seed=0, max_iterations=10000,
+           tree_capacity=65536, lanes=64, out_cap=4096):
+    """CPRRTC's planner run sequentially, as the GPU planner runs one query with one team
+    (exclusive mode; lanes 64: one iteration per pass, 32: two); see orc_cprrtc.  Returns a dict
+    with the keys of the GPU planner's result that the restatement fixes: status, path,
+    iterations, start_tree, goal_tree (and path_len, the length of a path too long for out_cap)."""
+    start = np.ascontiguousarray(start, np.float32)
+    goal = np.ascontiguousarray(goal, np.float32)
+    out = np.zeros((out_cap, robot.dof), np.float32)
+    inf = CprrtcInfo()
+    k = lib().orc_cprrtc(robot.h, scene.h, _p(start), _p(goal), resolution, max_step, seed, max_iterations,
+                         tree_capacity, lanes, _p(out), out_cap, C.byref(inf))
+    return {"status": CPRRTC_STATUS[inf.status], "path": out[:k].copy(), "iterations": inf.iterations,
+            "start_tree": inf.start_tree, "goal_tree": inf.goal_tree, "path_len": inf.path_len}
 
 
 def rrtconnect(robot, scene, start, goal, range_=0.3, resolution=0.02, seed=0, max_iterations=1_000_000,

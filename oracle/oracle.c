@@ -1108,3 +1108,195 @@ int orc_rrtconnect(const orc_robot* r, const orc_scene* s, const float* start, c
     if (stats) *stats = st;
     return written;
 }
+
+
+/* ----------------------------------------------------------------------------------------- */
+/* CPRRTC planner, one team per query                                                        */
+/* ----------------------------------------------------------------------------------------- */
+/* The parallel RRT-Connect of CPRRTC (RobotSolver.cu:507-799) run as the GPU planner runs it
+ * with one team per query (exclusive mode), which makes it sequential and deterministic per seed.
+ *
+ * One iteration (64 lanes): draw sample i of the query's Philox stream (orc_sample); grow the
+ * smaller tree, the start tree on a tie (RobotSolver.cu:570-588), from its node nearest to the
+ * sample (squared distance accumulated with fmaf over the joints, ties to the earlier node).  The
+ * extension covers ms = min(floor(d / res), max_step) states from that node toward the sample,
+ * spaced res apart (state k = fmaf(delta, k, from), delta = ((sample - from) / d) * res); with
+ * ms = 0 nothing happens.  Every state is checked and the last one becomes the new node.  Then
+ * the same tree connects greedily toward the other tree's node nearest to the new node: while
+ * the gap is at least 2 res, a chunk of min(floor(gap / res), max_step) states from the current
+ * node, inserted at its last state when all are free, else the connection stops; a gap below
+ * 2 res joins the trees unchecked (RobotSolver.cu:686-794).
+ *
+ * Paired iterations (32 lanes, the reference's 32-thread blocks two to a wave): samples 2p and
+ * 2p + 1 (the second only within the budget) run one iteration each against the tree sizes read
+ * before the pair, so both grow the same tree and search the same nodes; their extensions are
+ * inserted first's first, and their connections advance in lockstep, one chunk each per step,
+ * inserts in the same order; the first connection to close (the first sample's when both close
+ * in one step) solves the query.
+ *
+ * The path is the start tree's branch from its root to the joining node, then the goal tree's
+ * from its joining node to its root (RobotSolver.cu:297-346).  Statuses as MP_PLAN_*: 1 solved,
+ * 2 exhausted, 3 tree full (an insert beyond cap nodes), 4 invalid start or goal.  iterations =
+ * min(samples drawn, max_iters). */
+static int tree_nearest(const tree_t* t, int cnt, const float* q, float* best_out) {
+    float best = INFINITY;
+    int bi = 0;
+    for (int i = 0; i < cnt; i++) {
+        float acc = 0.0f;
+        for (int j = 0; j < t->dof; j++) {
+            float d0 = q[j] - t->q[i * t->dof + j];
+            acc = fmaf(d0, d0, acc);
+        }
+        if (acc < best) { best = acc; bi = i; }
+    }
+    *best_out = best;
+    return bi;
+}
+
+static int chunk_free(const orc_robot* r, const orc_scene* s, const float* from, const float* delta, int n) {
+    float q[MAXDOF];
+    for (int k = 0; k < n; k++) {
+        for (int j = 0; j < r->dof; j++) q[j] = fmaf(delta[j], (float)k, from[j]);
+        if (!check_q(r, s, q)) return 0;
+    }
+    return 1;
+}
+
+typedef struct {
+    int live, parent, onn;
+    float cur[MAXDOF], dcur;
+} cp_branch;
+
+/* The greedy connections of the live branches, in lockstep; returns the status (0 running). */
+static int cp_connect(const orc_robot* r, const orc_scene* s, tree_t* T, int tgt, cp_branch* b, int nb, float res,
+                      int max_step, int cap, int* join) {
+    const int dof = r->dof, oth = 1 - tgt;
+    for (;;) {
+        int solved = 0, live = 0;
+        for (int h = 0; h < nb; h++) {
+            if (!b[h].live || !(b[h].dcur < 2.0f * res)) continue;
+            if (!solved) {
+                join[tgt] = b[h].parent;
+                join[oth] = b[h].onn;
+                solved = 1;
+            }
+            b[h].live = 0;
+        }
+        if (solved) return 1;
+        float n1[2][MAXDOF];
+        for (int h = 0; h < nb; h++) {
+            if (!b[h].live) continue;
+            const float* to = T[oth].q + b[h].onn * dof;
+            int ms1 = (int)(b[h].dcur / res);
+            ms1 = ms1 < max_step ? ms1 : max_step;
+            float d1v[MAXDOF];
+            for (int j = 0; j < dof; j++) {
+                d1v[j] = ((to[j] - b[h].cur[j]) / b[h].dcur) * res;
+                n1[h][j] = fmaf(d1v[j], (float)(ms1 - 1), b[h].cur[j]);
+            }
+            if (!chunk_free(r, s, b[h].cur, d1v, ms1)) b[h].live = 0;  /* trapped */
+            live += b[h].live;
+        }
+        if (!live) return 0;
+        for (int h = 0; h < nb; h++) {
+            if (!b[h].live) continue;
+            if (T[tgt].n >= cap) return 3;
+            b[h].parent = tree_add(&T[tgt], n1[h], b[h].parent);
+        }
+        for (int h = 0; h < nb; h++) {
+            if (!b[h].live) continue;
+            const float* to = T[oth].q + b[h].onn * dof;
+            memcpy(b[h].cur, n1[h], sizeof(float) * dof);
+            float acc = 0.0f;
+            for (int j = 0; j < dof; j++) {
+                float df = to[j] - n1[h][j];
+                acc = fmaf(df, df, acc);
+            }
+            b[h].dcur = sqrtf(acc);
+        }
+    }
+}
+
+int orc_cprrtc(const orc_robot* r, const orc_scene* s, const float* start, const float* goal, float res,
+               int max_step, uint64_t seed, int max_iters, int cap, int lanes, float* out_path, int out_cap,
+               orc_cprrtc_info* info) {
+    const int dof = r->dof;
+    const int step = lanes == 32 ? 2 : 1;
+    orc_cprrtc_info inf;
+    memset(&inf, 0, sizeof(inf));
+    if (!check_q(r, s, start) || !check_q(r, s, goal)) {
+        inf.status = 4;
+        inf.start_tree = inf.goal_tree = 1;
+        if (info) *info = inf;
+        return 0;
+    }
+    tree_t T[2];
+    tree_init(&T[0], dof, start);
+    tree_init(&T[1], dof, goal);
+    int status = 0, join[2] = {-1, -1};
+    long drawn = 0;
+    for (int sidx = 0; status == 0; sidx += step) {
+        drawn = (long)sidx + step;
+        if (sidx >= max_iters) { status = 2; break; }
+        const int nb = step == 2 && sidx + 1 < max_iters ? 2 : 1;
+        const int tgt = T[0].n > T[1].n ? 1 : 0, oth = 1 - tgt;
+        const int cnt = T[tgt].n;
+        cp_branch b[2];
+        float node[2][MAXDOF];
+        int nn[2], live = 0;
+        for (int h = 0; h < nb; h++) {
+            float smp[MAXDOF], delta[MAXDOF], bd;
+            orc_sample(r, seed, (uint64_t)(sidx + h), smp);
+            nn[h] = tree_nearest(&T[tgt], cnt, smp, &bd);
+            const float d = sqrtf(bd);
+            int ms = (int)(d / res);
+            ms = ms < max_step ? ms : max_step;
+            b[h].live = ms > 0;
+            if (!b[h].live) continue;
+            const float* from = T[tgt].q + nn[h] * dof;
+            for (int j = 0; j < dof; j++) delta[j] = ((smp[j] - from[j]) / d) * res;
+            b[h].live = chunk_free(r, s, from, delta, ms);
+            for (int j = 0; j < dof; j++) node[h][j] = fmaf(delta[j], (float)(ms - 1), from[j]);
+            live += b[h].live;
+        }
+        if (!live) continue;
+        for (int h = 0; h < nb && status == 0; h++) {
+            if (!b[h].live) continue;
+            if (T[tgt].n >= cap) status = 3;
+            else b[h].parent = tree_add(&T[tgt], node[h], nn[h]);
+        }
+        if (status) break;
+        for (int h = 0; h < nb; h++) {
+            if (!b[h].live) continue;
+            float od;
+            b[h].onn = tree_nearest(&T[oth], T[oth].n, node[h], &od);
+            memcpy(b[h].cur, node[h], sizeof(float) * dof);
+            b[h].dcur = sqrtf(od);
+        }
+        status = cp_connect(r, s, T, tgt, b, nb, res, max_step, cap, join);
+    }
+    inf.status = status;
+    inf.iterations = (int)(drawn < max_iters ? drawn : max_iters);
+    inf.start_tree = T[0].n;
+    inf.goal_tree = T[1].n;
+    int written = 0;
+    if (status == 1) {
+        int n1 = 0, n2 = 0;
+        for (int v = join[0]; v >= 0; v = T[0].parent[v]) n1++;
+        for (int v = join[1]; v >= 0; v = T[1].parent[v]) n2++;
+        inf.path_len = n1 + n2;
+        if (n1 + n2 <= out_cap) {
+            int k = n1 - 1;
+            for (int v = join[0]; v >= 0; v = T[0].parent[v], k--)
+                memcpy(out_path + k * dof, T[0].q + v * dof, sizeof(float) * dof);
+            k = n1;
+            for (int v = join[1]; v >= 0; v = T[1].parent[v], k++)
+                memcpy(out_path + k * dof, T[1].q + v * dof, sizeof(float) * dof);
+            written = n1 + n2;
+        }
+    }
+    tree_free(&T[0]);
+    tree_free(&T[1]);
+    if (info) *info = inf;
+    return written;
+}

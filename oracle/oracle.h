@@ -147,6 +147,18 @@ int orc_rrtconnect(const orc_robot* r, const orc_scene* s, const float* start, c
                    float range, float resolution, uint64_t seed, int max_iterations, double time_limit,
                    float* out_path, int out_cap, orc_plan_stats* stats);
 
+/* CPRRTC's planner as the GPU planner runs it with one team per query (exclusive mode), lanes 64
+ * (one iteration per pass) or 32 (two per pass): sequential, deterministic per seed (orc_sample's
+ * stream).  Returns the number of waypoints written (0 unless solved and the path fits out_cap). */
+typedef struct {
+    int status;      /* 1 solved, 2 exhausted, 3 tree full, 4 invalid start or goal */
+    int iterations;  /* samples drawn */
+    int start_tree, goal_tree, path_len;
+} orc_cprrtc_info;
+int orc_cprrtc(const orc_robot* r, const orc_scene* s, const float* start, const float* goal, float res,
+               int max_step, uint64_t seed, int max_iters, int cap, int lanes, float* out_path, int out_cap,
+               orc_cprrtc_info* info);
+
 #ifdef __cplusplus
 }
 #endif

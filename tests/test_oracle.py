@@ -575,6 +575,97 @@ def test_rrtconnect_path_invariants(rng):
     assert solved >= 10
 
 
+# ------------------------------------------------------------------------------ CPRRTC planner
+def _cprrtc_chain(x, g, res, max_step):
+    """Greedy connection from x to g in float64: the nodes inserted, one per chunk of
+    min(floor(gap / res), max_step) states (the chunk's last state), until the gap is below 2 res."""
+    cur, out = x.copy(), []
+    while np.linalg.norm(g - cur) >= 2 * res:
+        d = np.linalg.norm(g - cur)
+        ms = min(int(d / res), max_step)
+        cur = cur + (g - cur) / d * res * (ms - 1)
+        out.append(cur)
+    return out
+
+
+def _cprrtc_extend(s, u, res, max_step):
+    d = np.linalg.norm(u - s)
+    ms = min(int(d / res), max_step)
+    return s + (u - s) / d * res * (ms - 1)
+
+
+@pytest.mark.parametrize("lanes", [64, 32])
+def test_cprrtc_free_space_closed_form(lanes):
+    """Free space: the first pass solves.  Sample 0 (and sample 1 in a pair) extends the start tree
+    (the smaller tree, start on a tie) by ms - 1 resolution steps, ms = min(floor(d / res), 32);
+    each new node then connects to the goal in chunks of at most 31 steps until within 2 res.  In
+    a pair both connections advance one chunk per step and the one that closes first (the first
+    sample's on a tie) gives the path; every node of both is in the start tree."""
+    m = point_robot(0.02, lim=1.0)
+    ro, so = oracle.OracleRobot(m), oracle.OracleScene(Scene())
+    res, ms_max = 0.02, 32
+    for seed, (s, g) in enumerate([([-0.5, 0.0, 0.0], [0.5, 0.0, 0.0]), ([0.1, 0.2, 0.3], [0.2, 0.1, -0.3]),
+                                   ([-0.9, -0.9, -0.9], [0.9, 0.9, 0.9]), ([0.0, 0.5, 0.0], [0.0, -0.5, 0.2])]):
+        s, g = np.array(s, np.float32), np.array(g, np.float32)
+        r = oracle.cprrtc(ro, so, s, g, seed=seed, lanes=lanes)
+        u = ro.sample(seed, 0, 2).astype(np.float64)
+        s64, g64 = s.astype(np.float64), g.astype(np.float64)
+        xs = [_cprrtc_extend(s64, u[h], res, ms_max) for h in range(1 if lanes == 64 else 2)]
+        chains = [_cprrtc_chain(x, g64, res, ms_max) for x in xs]
+        win = int(np.argmin([len(c) for c in chains]))  # argmin keeps the first on a tie
+        want = np.array([s64, xs[win]] + chains[win] + [g64])
+        assert r["status"] == "solved" and r["iterations"] == (1 if lanes == 64 else 2)
+        assert r["goal_tree"] == 1 and r["start_tree"] == 1 + len(xs) + len(xs) * len(chains[win])
+        assert r["path"].shape == want.shape, (r["path"].shape, want.shape)
+        assert np.allclose(r["path"], want, atol=1e-5)
+        assert np.array_equal(r["path"][0], s) and np.array_equal(r["path"][-1], g)
+
+
+@pytest.mark.parametrize("lanes", [64, 32])
+def test_cprrtc_wall_paths_keep_the_contract(lanes):
+    """Around a wall: every path starts and ends at the query; each tree edge is at most 31
+    resolution steps long and free when re-checked at the resolution (sphere 1e-5 m smaller, so
+    that float64 re-interpolation cannot flip a state on a contact boundary), except the joining
+    gap, below 2 res; a seed gives the same result twice, and different seeds differ."""
+    m, shrunk = point_robot(0.02, lim=1.0), point_robot(0.02 - 1e-5, lim=1.0)
+    ro, rs = oracle.OracleRobot(m), oracle.OracleRobot(shrunk)
+    so = oracle.OracleScene(Scene(boxes=box_record([0, 0, 0], np.eye(3), [0.05, 0.6, 0.6])[None]))
+    s, g = np.array([-0.5, 0, 0], np.float32), np.array([0.5, 0, 0], np.float32)
+    paths = []
+    for seed in range(8):
+        r = oracle.cprrtc(ro, so, s, g, seed=seed, lanes=lanes)
+        assert r["status"] == "solved" and 1 <= r["iterations"] <= 10000
+        p = r["path"]
+        assert np.array_equal(p[0], s) and np.array_equal(p[-1], g) and len(p) >= 3
+        seg = np.linalg.norm(np.diff(p.astype(np.float64), axis=0), axis=1)
+        assert np.all(seg <= 31 * 0.02 * (1 + 1e-5)), seg.max()
+        assert int(np.sum(seg < 2 * 0.02)) >= 1  # the joining gap
+        assert oracle.validate_path(rs, so, p, 0.02)[0]
+        assert len(p) <= r["start_tree"] + r["goal_tree"]
+        again = oracle.cprrtc(ro, so, s, g, seed=seed, lanes=lanes)
+        assert np.array_equal(again["path"], p) and again["iterations"] == r["iterations"]
+        paths.append(p)
+    assert len({p.tobytes() for p in paths}) == len(paths)
+
+
+def test_cprrtc_statuses():
+    """Invalid endpoints (no sample drawn), a tree capacity of 2 (full at the first insert past
+    it), and budgets of 1 and 3 samples around a wall no single pass can cross (a pass of 32
+    lanes draws two samples, the second only within the budget)."""
+    m = point_robot(0.02, lim=1.0)
+    ro = oracle.OracleRobot(m)
+    so = oracle.OracleScene(Scene(boxes=box_record([0, 0, 0], np.eye(3), [0.05, 0.6, 0.6])[None]))
+    s, g = np.array([-0.5, 0, 0], np.float32), np.array([0.5, 0, 0], np.float32)
+    for lanes in (64, 32):
+        r = oracle.cprrtc(ro, so, np.zeros(3, np.float32), g, lanes=lanes)
+        assert (r["status"], r["iterations"], r["start_tree"], r["goal_tree"]) == ("invalid", 0, 1, 1)
+        r = oracle.cprrtc(ro, so, s, g, tree_capacity=2, lanes=lanes)
+        assert r["status"] == "tree_full" and max(r["start_tree"], r["goal_tree"]) == 2 and not len(r["path"])
+        for budget in (1, 3):
+            r = oracle.cprrtc(ro, so, s, g, seed=5, max_iterations=budget, lanes=lanes)
+            assert r["status"] == "exhausted" and r["iterations"] == budget
+
+
 # ------------------------------------------------------------------------------------- roadmap
 def test_rrg_free_space_closed_form():
     """Free space: every sample of the first pass reaches both groups, so the first pass solves.
