@@ -214,12 +214,7 @@ void run_queries(Module& mod, const RunBuffers& b, int nq, const mp_rrtc_params&
     h[4] = 0;       // first new
     h[5] = nq;      // new count
     h[6] = p.exclusive ? 1 : 0;
-    static const int per_ticket = [] {
-        const char* e = std::getenv("MP_ITERS_PER_TICKET");  // experiment override
-        const int v = e ? std::atoi(e) : 0;
-        return v > 0 ? v : kItersPerTicket;
-    }();
-    h[7] = per_ticket;
+    h[7] = kItersPerTicket;
     h[8] = cap;
     h[9] = pair_flag(lanes);
     for (int k = 0; k < nq; k++) h[16 + k] = k;
