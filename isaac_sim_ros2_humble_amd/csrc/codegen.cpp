@@ -891,6 +891,7 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "    const int per_ticket = mp_uniform(ctl.ctl[7]) > 0 ? mp_uniform(ctl.ctl[7]) : 1;\n"
          "    const int cap = mp_uniform(ctl.ctl[8]);\n"
          "    const bool pair = mp_uniform(ctl.ctl[9]) != 0;  // two iterations per pass (32 lanes each)\n"
+         "    const int cu_slot = mp_cu_slot();\n"
          "    if (n_active <= 0) return;\n"
       << team_setup <<
          "    const bool lead = T.w == 0;\n"
@@ -919,8 +920,9 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "        return;\n    }\n"
          "    for (;;) {\n"
          "        u64 ts = mp_now(pl);\n"
-         "        // the lead wave takes a ticket and picks the query: round-robin over the active list,\n"
-         "        // skipping finished queries with a wave-wide probe (-2: the launch is done)\n"
+         "        // the lead wave takes a ticket and picks the query: its CU's query when that one runs\n"
+         "        // below the cap, else round-robin over the active list, skipping finished queries with\n"
+         "        // a wave-wide probe (-2: the launch is done)\n"
          "        int qi = -2;\n"
          "        if (lead) {\n"
          "            u32 t = 0;\n"
@@ -933,13 +935,25 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "            if (!(budget > 0 && t >= (u32)budget) && running > 0) {\n"
          "                qi = -1;\n"
          "                bool full = false;  // a running query was passed over: it had its teams\n"
+         "                const int home = mp_uniform(__hip_atomic_load(&mp_affinity[cu_slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));\n"
+         "                if (home >= 0 && home < n_active) {\n"
+         "                    const int cand = mp_uniform(ctl.active[home]);\n"
+         "                    if (mp_uniform(__hip_atomic_load(&ctl.st[cand].status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == MP_PLAN_RUNNING) {\n"
+         "                        int old = 0;\n"
+         "                        if (capq > 0 && lane == 0) old = __hip_atomic_fetch_add(&ctl.st[cand].workers, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "                        old = __shfl(old, 0);\n"
+         "                        if (capq <= 0 || old < capq) qi = cand;\n"
+         "                        else if (lane == 0) __hip_atomic_fetch_add(&ctl.st[cand].workers, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "                    }\n"
+         "                }\n"
+         "                int taken = -1;  // active-list position of a query the probe below takes\n"
          "                const int slot = (int)(t % (u32)n_active);\n"
          "                for (int base = 0; base < n_active && qi < 0; base += MP_WAVE) {\n"
          "                    const int p = base + lane;\n"
-         "                    int a = -1;\n"
+         "                    int a = -1, k = 0;\n"
          "                    bool run = false;\n"
          "                    if (p < n_active) {\n"
-         "                        int k = slot + p;\n"
+         "                        k = slot + p;\n"
          "                        if (k >= n_active) k -= n_active;\n"
          "                        a = ctl.active[k];\n"
          "                        run = __hip_atomic_load(&ctl.st[a].status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == MP_PLAN_RUNNING;\n"
@@ -947,17 +961,20 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "                    u64 b = __ballot(run);\n"
          "                    while (b) {\n"
          "                        const int cand = __shfl(a, __builtin_ctzll(b));\n"
-         "                        if (capq <= 0) { qi = cand; break; }\n"
+         "                        const int pos = __shfl(k, __builtin_ctzll(b));\n"
+         "                        if (capq <= 0) { qi = cand; taken = pos; break; }\n"
          "                        int old = 0;\n"
          "                        if (lane == 0) old = __hip_atomic_fetch_add(&ctl.st[cand].workers, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
          "                        old = __shfl(old, 0);\n"
-         "                        if (old < capq) { qi = cand; break; }\n"
+         "                        if (old < capq) { qi = cand; taken = pos; break; }\n"
          "                        if (lane == 0) __hip_atomic_fetch_add(&ctl.st[cand].workers, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
          "                        full = true;\n"
          "                        b &= b - 1ull;\n"
          "                    }\n"
          "                }\n"
          "                qi = mp_uniform(qi);\n"
+         "                if (taken >= 0 && lane == 0)\n"
+         "                    __hip_atomic_store(&mp_affinity[cu_slot], taken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
          "                // every running query it saw has its teams: this team is surplus and leaves\n"
          "                if (qi < 0 && full) qi = -2;\n"
 
