@@ -566,7 +566,7 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
     // kind: 0 box, 1 sphere, 2 capsule, 3 cylinder.  The obstacle parameters are named
     // m0..m11 lo{x,y,z} hi{x,y,z} (box), ox oy oz orad (sphere), x1 y1 z1 xv yv zv rdv crad
     // (capsule), m0..m11 crad hh (cylinder) in the generated loops.
-    auto env_block = [&](int kind, bool cost) {
+    auto env_block = [&](int kind, bool cost, const char* detail) {
         Emitter X;
         V mm[12];
         for (int i = 0; i < 12; i++) mm[i] = V{false, 0.0f, "m" + std::to_string(i)};
@@ -588,7 +588,9 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
                 any += (gi ? " | gb" : "gb") + std::to_string(gi);
             }
             X.os << "            work += " << kSph * (int)groups.size() << "u;\n";
-            X.os << "            if (" << any << ") {\n";
+            X.os << "            if (" << any << ") {\n" << detail;
+        } else {
+            X.os << detail;
         }
         for (size_t gi = 0; gi < groups.size(); gi++) {
         if (cull_group) X.os << "            {\n            if (gb" << gi << ") {\n";
@@ -728,42 +730,50 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
       << self_body(false) << "        return h;\n    }\n";
     o << "    static __device__ __forceinline__ float self_cost(const float* F, unsigned& work) {\n        float part = 0.0f;\n"
       << self_body(true) << "        return part;\n    }\n";
+    // an obstacle's bounding sphere first (its own section of the scene), its record only when a
+    // link group is near it
     const char* box_load =
+        "            const float* bb = sc.bbox + o * 4;\n"
+        "            const float ocx = mp_sld(bb + 0), ocy = mp_sld(bb + 1), ocz = mp_sld(bb + 2), orr = mp_sld(bb + 3);\n";
+    const char* box_detail =
         "            const float* b = sc.box + o * MP_BOX_W;\n"
         "            const float m0 = mp_sld(b + 0), m1 = mp_sld(b + 1), m2 = mp_sld(b + 2), m3 = mp_sld(b + 3);\n"
         "            const float m4 = mp_sld(b + 4), m5 = mp_sld(b + 5), m6 = mp_sld(b + 6), m7 = mp_sld(b + 7);\n"
         "            const float m8 = mp_sld(b + 8), m9 = mp_sld(b + 9), m10 = mp_sld(b + 10), m11 = mp_sld(b + 11);\n"
         "            const float lox = mp_sld(b + 12), loy = mp_sld(b + 13), loz = mp_sld(b + 14);\n"
-        "            const float hix = mp_sld(b + 15), hiy = mp_sld(b + 16), hiz = mp_sld(b + 17);\n"
-        "            const float ocx = mp_sld(b + 18), ocy = mp_sld(b + 19), ocz = mp_sld(b + 20), orr = mp_sld(b + 21);\n";
+        "            const float hix = mp_sld(b + 15), hiy = mp_sld(b + 16), hiz = mp_sld(b + 17);\n";
     const char* sph_load =
         "            const float* p = sc.sph + o * MP_SPH_W;\n"
         "            const float ox = mp_sld(p + 0), oy = mp_sld(p + 1), oz = mp_sld(p + 2), orad = mp_sld(p + 3);\n";
     const char* cap_load =
+        "            const float* kb = sc.bcap + o * 4;\n"
+        "            const float ocx = mp_sld(kb + 0), ocy = mp_sld(kb + 1), ocz = mp_sld(kb + 2), orr = mp_sld(kb + 3);\n";
+    const char* cap_detail =
         "            const float* k = sc.cap + o * MP_CAP_W;\n"
         "            const float x1 = mp_sld(k + 0), y1 = mp_sld(k + 1), z1 = mp_sld(k + 2);\n"
         "            const float xv = mp_sld(k + 3), yv = mp_sld(k + 4), zv = mp_sld(k + 5);\n"
-        "            const float rdv = mp_sld(k + 6), crad = mp_sld(k + 7);\n"
-        "            const float ocx = mp_sld(k + 8), ocy = mp_sld(k + 9), ocz = mp_sld(k + 10), orr = mp_sld(k + 11);\n";
+        "            const float rdv = mp_sld(k + 6), crad = mp_sld(k + 7);\n";
     const char* cyl_load =
+        "            const float* cb = sc.bcyl + o * 4;\n"
+        "            const float ocx = mp_sld(cb + 0), ocy = mp_sld(cb + 1), ocz = mp_sld(cb + 2), orr = mp_sld(cb + 3);\n";
+    const char* cyl_detail =
         "            const float* b = sc.cyl + o * MP_CYL_W;\n"
         "            const float m0 = mp_sld(b + 0), m1 = mp_sld(b + 1), m2 = mp_sld(b + 2), m3 = mp_sld(b + 3);\n"
         "            const float m4 = mp_sld(b + 4), m5 = mp_sld(b + 5), m6 = mp_sld(b + 6), m7 = mp_sld(b + 7);\n"
         "            const float m8 = mp_sld(b + 8), m9 = mp_sld(b + 9), m10 = mp_sld(b + 10), m11 = mp_sld(b + 11);\n"
-        "            const float crad = mp_sld(b + 12), hh = mp_sld(b + 13);\n"
-        "            const float ocx = mp_sld(b + 14), ocy = mp_sld(b + 15), ocz = mp_sld(b + 16), orr = mp_sld(b + 17);\n";
+        "            const float crad = mp_sld(b + 12), hh = mp_sld(b + 13);\n";
     // early-out order: boxes, spheres, capsules, cylinders (cheapest per pair first)
     // in a team, wave w takes the obstacles o with o % W == w of every kind; the team's early-exit
     // flag (an LDS read) is looked at every fourth obstacle
     o << "    template <int MODE>\n    static __device__ __forceinline__ bool env_hit(const float* F, const SceneDev& sc, bool hit, u64 mask, unsigned& work, const Team& T) {\n";
     o << "        const int o0 = mp_uniform(T.w), os = mp_uniform(T.W);\n" << group_prologue;
-    o << "        for (int o = o0, it = 1; o < mp_uniform(sc.nb); o += os, ++it) {\n" << box_load << env_block(0, false)
+    o << "        for (int o = o0, it = 1; o < mp_uniform(sc.nb); o += os, ++it) {\n" << box_load << env_block(0, false, box_detail)
       << "            if (mp_stop<MODE>(hit, mask) || ((it & 3) == 0 && T.stopped())) return hit;\n        }\n";
-    o << "        for (int o = o0, it = 1; o < mp_uniform(sc.ns); o += os, ++it) {\n" << sph_load << env_block(1, false)
+    o << "        for (int o = o0, it = 1; o < mp_uniform(sc.ns); o += os, ++it) {\n" << sph_load << env_block(1, false, "")
       << "            if (mp_stop<MODE>(hit, mask) || ((it & 3) == 0 && T.stopped())) return hit;\n        }\n";
-    o << "        for (int o = o0, it = 1; o < mp_uniform(sc.ncap); o += os, ++it) {\n" << cap_load << env_block(2, false)
+    o << "        for (int o = o0, it = 1; o < mp_uniform(sc.ncap); o += os, ++it) {\n" << cap_load << env_block(2, false, cap_detail)
       << "            if (mp_stop<MODE>(hit, mask) || ((it & 3) == 0 && T.stopped())) return hit;\n        }\n";
-    o << "        for (int o = o0, it = 1; o < mp_uniform(sc.nc); o += os, ++it) {\n" << cyl_load << env_block(3, false)
+    o << "        for (int o = o0, it = 1; o < mp_uniform(sc.nc); o += os, ++it) {\n" << cyl_load << env_block(3, false, cyl_detail)
       << "            if (mp_stop<MODE>(hit, mask) || ((it & 3) == 0 && T.stopped())) return hit;\n        }\n";
     o << "        return hit;\n    }\n";
     // full-evaluation costs in the oracle order: spheres, boxes, cylinders, capsules; culled
@@ -772,13 +782,13 @@ std::string robot_struct_source(const RobotModel& m, const std::string& sname) {
     // capsule constraints, States.h:205's [constraint][state] rows)
     o << "    static __device__ __forceinline__ void env_cost(const float* F, const SceneDev& sc, unsigned& work, float* pk) {\n";
     o << "        float part = 0.0f;\n" << group_prologue;
-    o << "        for (int o = 0; o < mp_uniform(sc.ns); ++o) {\n" << sph_load << env_block(1, true) << "        }\n";
+    o << "        for (int o = 0; o < mp_uniform(sc.ns); ++o) {\n" << sph_load << env_block(1, true, "") << "        }\n";
     o << "        pk[0] = part; part = 0.0f;\n";
-    o << "        for (int o = 0; o < mp_uniform(sc.nb); ++o) {\n" << box_load << env_block(0, true) << "        }\n";
+    o << "        for (int o = 0; o < mp_uniform(sc.nb); ++o) {\n" << box_load << env_block(0, true, box_detail) << "        }\n";
     o << "        pk[1] = part; part = 0.0f;\n";
-    o << "        for (int o = 0; o < mp_uniform(sc.nc); ++o) {\n" << cyl_load << env_block(3, true) << "        }\n";
+    o << "        for (int o = 0; o < mp_uniform(sc.nc); ++o) {\n" << cyl_load << env_block(3, true, cyl_detail) << "        }\n";
     o << "        pk[2] = part; part = 0.0f;\n";
-    o << "        for (int o = 0; o < mp_uniform(sc.ncap); ++o) {\n" << cap_load << env_block(2, true) << "        }\n";
+    o << "        for (int o = 0; o < mp_uniform(sc.ncap); ++o) {\n" << cap_load << env_block(2, true, cap_detail) << "        }\n";
     o << "        pk[3] = part;\n    }\n";
     // joint bounds: BoundaryConstraint.cu:56-92, tolerance 1e-4 on the squared sum
     o << "    static __device__ __forceinline__ float bounds_acc(const float* q) {\n        float acc = 0.0f;\n";
@@ -868,6 +878,7 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "    Q.scene.ncap = mp_sld(&p->scene.ncap);\n"
          "    Q.scene.sph = mp_sld(&p->scene.sph); Q.scene.box = mp_sld(&p->scene.box); Q.scene.cyl = mp_sld(&p->scene.cyl);\n"
          "    Q.scene.cap = mp_sld(&p->scene.cap); Q.scene.bnd = mp_sld(&p->scene.bnd);\n"
+         "    Q.scene.bbox = mp_sld(&p->scene.bbox); Q.scene.bcyl = mp_sld(&p->scene.bcyl); Q.scene.bcap = mp_sld(&p->scene.bcap);\n"
          "    Q.tree[0] = mp_sld(&p->tree[0]); Q.tree[1] = mp_sld(&p->tree[1]);\n"
          "    Q.parent[0] = mp_sld(&p->parent[0]); Q.parent[1] = mp_sld(&p->parent[1]); Q.path = mp_sld(&p->path);\n"
          "    Q.seed = mp_sld(&p->seed); Q.robot = mp_sld(&p->robot); Q.dof = mp_sld(&p->dof); Q.cap = mp_sld(&p->cap);\n"

@@ -84,9 +84,12 @@ SceneDevHost scene_view(const mp_scene* sc) {
     v.nc = sc->nc;
     v.ncap = sc->ncap;
     v.sph = sc->dev + sc->offs[0];
-    v.box = sc->dev + sc->offs[1];
-    v.cyl = sc->dev + sc->offs[2];
-    v.cap = sc->dev + sc->offs[3];
+    v.bbox = sc->dev + sc->offs[1];
+    v.box = sc->dev + sc->offs[2];
+    v.bcyl = sc->dev + sc->offs[3];
+    v.cyl = sc->dev + sc->offs[4];
+    v.bcap = sc->dev + sc->offs[5];
+    v.cap = sc->dev + sc->offs[6];
     v.bnd = sc->bounds_mode == MP_BOUNDS_ROBOT ? nullptr : sc->bnd_dev;
     return v;
 }
@@ -448,10 +451,10 @@ void bound_sphere(float* o, double cx, double cy, double cz, double r) {
 
 void scene_upload(mp_scene* sc, const float* spheres, int ns, const float* boxes, int nb, const float* cyl, int nc,
                   const float* caps, int ncap) {
-    std::vector<float> h(sc->offs[3] + (size_t)MP_CAP_W_H * std::max(sc->cap_k, 1), 0.0f);
+    std::vector<float> h(sc->offs[6] + (size_t)MP_CAP_W_H * std::max(sc->cap_k, 1), 0.0f);
     for (int i = 0; i < ns; i++) std::memcpy(&h[sc->offs[0] + 4 * i], spheres + 4 * i, 4 * sizeof(float));
     for (int i = 0; i < nb; i++) {
-        float* o = &h[sc->offs[1] + MP_BOX_W_H * i];
+        float* o = &h[sc->offs[2] + MP_BOX_W_H * i];
         const float* b = boxes + 18 * i;
         inverse_pose(b, b + 3, o);
         std::memcpy(o + 12, b + 12, 6 * sizeof(float));
@@ -460,9 +463,10 @@ void scene_upload(mp_scene* sc, const float* spheres, int ns, const float* boxes
         double wc[3], rho;
         local_box_bound(o, b + 12, b + 15, wc, rho);
         bound_sphere(o + 18, wc[0], wc[1], wc[2], rho);
+        std::memcpy(&h[sc->offs[1] + 4 * i], o + 18, 4 * sizeof(float));
     }
     for (int i = 0; i < nc; i++) {
-        float* o = &h[sc->offs[2] + MP_CYL_W_H * i];
+        float* o = &h[sc->offs[4] + MP_CYL_W_H * i];
         inverse_pose(cyl + 14 * i, cyl + 14 * i + 3, o);
         o[12] = cyl[14 * i + 12];
         o[13] = cyl[14 * i + 13] * 0.5f;
@@ -471,10 +475,11 @@ void scene_upload(mp_scene* sc, const float* spheres, int ns, const float* boxes
         double wc[3], rho;
         local_box_bound(o, lo, hi, wc, rho);
         bound_sphere(o + 14, wc[0], wc[1], wc[2], rho);
+        std::memcpy(&h[sc->offs[3] + 4 * i], o + 14, 4 * sizeof(float));
     }
     for (int i = 0; i < ncap; i++) {
         const float* c = caps + 7 * i;
-        float* o = &h[sc->offs[3] + MP_CAP_W_H * i];
+        float* o = &h[sc->offs[6] + MP_CAP_W_H * i];
         o[0] = c[0]; o[1] = c[1]; o[2] = c[2];
         o[3] = c[3] - c[0]; o[4] = c[4] - c[1]; o[5] = c[5] - c[2];
         o[6] = 1.0f / dot3f(o[3], o[3], o[4], o[4], o[5], o[5]);
@@ -482,6 +487,7 @@ void scene_upload(mp_scene* sc, const float* spheres, int ns, const float* boxes
         const double vx = (double)c[3] - c[0], vy = (double)c[4] - c[1], vz = (double)c[5] - c[2];
         bound_sphere(o + 8, 0.5 * ((double)c[0] + c[3]), 0.5 * ((double)c[1] + c[4]), 0.5 * ((double)c[2] + c[5]),
                      0.5 * std::sqrt(vx * vx + vy * vy + vz * vz) + c[6]);
+        std::memcpy(&h[sc->offs[5] + 4 * i], o + 8, 4 * sizeof(float));
     }
     MP_HIP(hipMemcpyAsync(sc->dev, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, G.stream));
     MP_HIP(hipStreamSynchronize(G.stream));
@@ -506,9 +512,12 @@ int mp_scene_create(const float* spheres, int ns, const float* boxes, int nb, co
         sc->cap_k = ncap;
         sc->offs[0] = 0;
         sc->offs[1] = sc->offs[0] + (size_t)4 * std::max(ns, 1);
-        sc->offs[2] = sc->offs[1] + (size_t)MP_BOX_W_H * std::max(nb, 1);
-        sc->offs[3] = sc->offs[2] + (size_t)MP_CYL_W_H * std::max(nc, 1);
-        size_t total = sc->offs[3] + (size_t)MP_CAP_W_H * std::max(ncap, 1);
+        sc->offs[2] = sc->offs[1] + (size_t)4 * std::max(nb, 1);
+        sc->offs[3] = sc->offs[2] + (size_t)MP_BOX_W_H * std::max(nb, 1);
+        sc->offs[4] = sc->offs[3] + (size_t)4 * std::max(nc, 1);
+        sc->offs[5] = sc->offs[4] + (size_t)MP_CYL_W_H * std::max(nc, 1);
+        sc->offs[6] = sc->offs[5] + (size_t)4 * std::max(ncap, 1);
+        size_t total = sc->offs[6] + (size_t)MP_CAP_W_H * std::max(ncap, 1);
         try {
             MP_HIP(hipMalloc(&sc->dev, total * sizeof(float)));
             scene_upload(sc, spheres, ns, boxes, nb, cylinders, nc, capsules, ncap);

@@ -96,8 +96,10 @@ struct mp_robot {
 struct mp_scene {
     int ns = 0, nb = 0, nc = 0, ncap = 0;
     int cap_s = 0, cap_b = 0, cap_c = 0, cap_k = 0;
-    float* dev = nullptr;  // one allocation: spheres | boxes | cylinders | capsules
-    size_t offs[4] = {0, 0, 0, 0};
+    // one allocation: spheres | box bounds | boxes | cylinder bounds | cylinders | capsule bounds
+    // | capsules, each section sized by the capacity of its kind
+    float* dev = nullptr;
+    size_t offs[7] = {0, 0, 0, 0, 0, 0, 0};
     // joint bounds of every check on this scene: MP_BOUNDS_ROBOT (the robot's limits), or the
     // 32 floats lower[16] | upper[16] in bnd_dev
     int bounds_mode = 0;

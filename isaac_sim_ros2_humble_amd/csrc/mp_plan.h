@@ -19,6 +19,9 @@ struct SceneDevHost {
     const float* cyl;
     const float* cap;
     const float* bnd;  // null: the robot's joint limits; else lower[16], upper[16]
+    const float* bbox;  // bounding spheres of the boxes, cylinders, capsules (4 floats each)
+    const float* bcyl;
+    const float* bcap;
 };
 
 struct QueryDevHost {
@@ -32,7 +35,7 @@ struct QueryDevHost {
     float start[16];
     float goal[16];
 };
-static_assert(sizeof(QueryDevHost) == 264, "QueryDev layout");
+static_assert(sizeof(QueryDevHost) == 288, "QueryDev layout");
 
 struct QueryStateHost {
     int alloc[2];

@@ -56,7 +56,8 @@ def test_query_descriptor_loads_every_field_the_kernels_read(fr3):
     body = src[src.index("mp_load_query("):]
     body = body[:body.index("\n}\n")]
     for f in ("tree[0]", "tree[1]", "parent[0]", "parent[1]", "path", "seed", "robot", "dof", "cap", "max_iters",
-              "max_path", "dof_max", "sampler", "scene.box", "scene.sph", "scene.cyl", "scene.cap", "scene.bnd"):
+              "max_path", "dof_max", "sampler", "scene.box", "scene.sph", "scene.cyl", "scene.cap", "scene.bnd", "scene.bbox", "scene.bcyl",
+              "scene.bcap"):
         assert "Q.%s = mp_sld(&p->%s)" % (f, f) in body, f
 
 
