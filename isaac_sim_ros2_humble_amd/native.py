@@ -529,7 +529,7 @@ class BatchResult:
         self._gen = req.generation
         r = req.paths.copy()
         self.code = r["status"]
-        self.status = [PLAN_STATUS.get(int(c), str(int(c))) for c in self.code]
+        self._status = None
         self.solved = int((self.code == 1).sum())
         self.num_waypoints = r["num_waypoints"]
         self.iterations = r["iterations"]
@@ -539,6 +539,13 @@ class BatchResult:
         self.latency_ms = r["latency_ms"]
         self.work_flops = r["work_flops"]
         self.device_ms = r["device_ms"]
+
+    @property
+    def status(self):
+        """Status names per query (from the codes, built on first use)."""
+        if self._status is None:
+            self._status = [PLAN_STATUS.get(c, str(c)) for c in self.code.tolist()]
+        return self._status
 
     def path(self, k):
         if self._req.generation != self._gen:
