@@ -938,15 +938,18 @@ std::string module_source(const std::vector<const RobotModel*>& robots) {
          "        if (lead) {\n"
          "            u32 t = 0;\n"
          "            if (lane == 0) t = __hip_atomic_fetch_add((u32*)ctl.ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "            // the running count and the CU's query are read while the ticket is in flight\n"
+         "            const int running0 = __hip_atomic_load(ctl.ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "            const int home0 = __hip_atomic_load(&mp_affinity[cu_slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
          "            t = (u32)__shfl((int)t, 0);\n"
-         "            const int running = mp_uniform(__hip_atomic_load(ctl.ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));\n"
+         "            const int running = mp_uniform(running0);\n"
          "            // the cap grows as queries finish, so the last ones share every team of the launch\n"
          "            const int fair = running > 0 ? (int)((gridDim.x + (unsigned)running - 1u) / (unsigned)running) : 1;\n"
          "            const int capq = cap > 0 ? (cap > fair ? cap : fair) : 0;\n"
          "            if (!(budget > 0 && t >= (u32)budget) && running > 0) {\n"
          "                qi = -1;\n"
          "                bool full = false;  // a running query was passed over: it had its teams\n"
-         "                const int home = mp_uniform(__hip_atomic_load(&mp_affinity[cu_slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));\n"
+         "                const int home = mp_uniform(home0);\n"
          "                if (home >= 0 && home < n_active) {\n"
          "                    const int cand = mp_uniform(ctl.active[home]);\n"
          "                    if (mp_uniform(__hip_atomic_load(&ctl.st[cand].status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == MP_PLAN_RUNNING) {\n"
